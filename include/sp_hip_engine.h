@@ -1,0 +1,52 @@
+/* Plain-pointer C ABI of the HIP phasing engine (device buffers, sizes,
+ * return codes; no library types beyond SpPhaser).  These entry points stand
+ * where the reference's CUDA glue sits:
+ *
+ *   spx_device_available      sp_cuda_get_device_type
+ *                             (reference include/spimage/cuda_util.h:101-109, src/cuda_util.c:10-29)
+ *   spx_fft_device            the cufftPlan/cufftExecC2C call sites
+ *                             (reference src/phasing.cu:47-221, src/phasing.c:716-720, src/fft.cu:14-38)
+ *   spx_phaser_time_iterations  the reference's speed tests
+ *                             (reference tests/PhasingTests.c:1068-1217), timed with HIP events
+ *                             on the engine's stream
+ *
+ * Every function returns 0 on success and a negative code on error; the text
+ * of the last error is available from spx_last_error().
+ */
+#ifndef SPX_SP_HIP_ENGINE_H
+#define SPX_SP_HIP_ENGINE_H 1
+
+#include "sp_phasing.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1 when a HIP device is usable, 0 otherwise */
+spimage_EXPORT int spx_device_available(void);
+spimage_EXPORT const char *spx_last_error(void);
+/* 1 when a line of length L can be transformed by the engine */
+spimage_EXPORT int spx_supported_length(int L);
+
+/* Unnormalised c2c transform of an (nz, ny, nx) x-fastest complex64 array in
+ * device memory; direction -1 forward, +1 inverse; d_in == d_out allowed.
+ * Runs on the library's stream and returns after completion. */
+spimage_EXPORT int spx_fft_device(void *d_in, void *d_out, int nx, int ny, int nz, int direction);
+
+/* Run `iterations` phasing iterations of the phaser's current algorithm
+ * (no support updates) and report the elapsed device time in milliseconds,
+ * measured with HIP events on the engine's stream. */
+spimage_EXPORT int spx_phaser_time_iterations(SpPhaser *ph, int iterations, float *ms);
+
+/* Device addresses of the phaser's current iterate g1, previous iterate g0
+ * and model before projection gp (complex64, image_size elements each). */
+spimage_EXPORT int spx_phaser_device_arrays(SpPhaser *ph, void **g1, void **g0, void **gp);
+
+/* Seed of the engine's counter-based generator used by
+ * sp_phaser_init_model(..., SpModelRandomPhases / SpModelRandomValues). */
+spimage_EXPORT void spx_phaser_set_seed(SpPhaser *ph, unsigned long long seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

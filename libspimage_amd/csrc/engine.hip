@@ -1,0 +1,927 @@
+// Host side of the HIP phasing engine plus its small helper kernels
+// (initialisation, reductions, radix select of the shrinkwrap threshold).
+#include "engine.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace spx {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+const char* last_error() { return g_err; }
+
+bool device_available() {
+  static int state = -1;
+  if (state < 0) {
+    int n = 0;
+    state = (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+    if (state) hipSetDevice(0);
+  }
+  return state == 1;
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables: per stage s of a length-L plan, entry j*(R-1)+(k-1) holds
+// exp(-2 pi i j k / (Ns R)), evaluated in double and rounded to float
+// ---------------------------------------------------------------------------
+const float2* twiddles(int L) {
+  static std::mutex mu;
+  static std::map<int, float2*> tables;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = tables.find(L);
+  if (it != tables.end()) return it->second;
+  const FftPlan p = make_plan(L);
+  std::vector<float2> h(p.twtotal > 0 ? p.twtotal : 1);
+  for (int s = 0; s < p.n; ++s) {
+    const int R = p.r[s], Ns = p.ns[s];
+    for (int j = 0; j < Ns; ++j)
+      for (int k = 1; k < R; ++k) {
+        const double a = -2.0 * M_PI * (double)j * (double)k / (double)(Ns * R);
+        h[p.twoff[s] + j * (R - 1) + (k - 1)] = make_float2((float)cos(a), (float)sin(a));
+      }
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, h.size() * sizeof(float2)) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  tables[L] = d;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+__global__ void k_fill_c64(float2* p, long n, float2 v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(unsigned* p, long n, unsigned v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// amplitudes as the projection reads them: unmeasured pixels get the NaN
+// pattern kUnmeasuredBits, measured NaNs the canonical quiet NaN
+__global__ void k_encode_amp(const float* amp, const uint8_t* flags, float* ampx, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float a = amp[i];
+    if (!(flags[i] & kMeasured)) a = __uint_as_float(kUnmeasuredBits);
+    else if (isnan(a)) a = __uint_as_float(0x7FC00000u);
+    ampx[i] = a;
+  }
+}
+
+__global__ void k_scale_copy(const float2* in, float2* out, long n, float s) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float2 v = in[i];
+    // sp_cmul(v, (s, 0)) as in sp_image_scale (reference src/image_util.c:1967-1970)
+    out[i] = make_float2(v.x * s - v.y * 0.f, v.x * 0.f + s * v.y);
+  }
+}
+
+__global__ void k_sub(const float2* a, const float2* b, float2* out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
+}
+
+// counter-based generator: splitmix64 finaliser of (seed, index, stream)
+__device__ __forceinline__ double rng_u01(uint64_t seed, uint64_t i, uint64_t stream) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (2 * i + stream + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+}
+
+// |amp| with a random (or zero) phase: sp_image_rephase (reference
+// src/image_util.c:1044-1067)
+__global__ void k_init_phases(const float* amp, float2* out, long n, uint64_t seed, int random) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float a = fabsf(amp[i]);
+    if (random) {
+      const float phase = (float)(rng_u01(seed, i, 0) * M_PI * 2.0);
+      out[i] = make_float2((float)(cos((double)phase) * a), (float)(sin((double)phase) * a));
+    } else {
+      out[i] = make_float2(a, 0.f);
+    }
+  }
+}
+
+__global__ void k_random_values(float2* out, long n, uint64_t seed) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)((float)rng_u01(seed, i, 0) - 0.5), (float)((float)rng_u01(seed, i, 1) - 0.5));
+}
+
+__global__ void k_amp2(const float* amp, float2* out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = make_float2(amp[i] * amp[i], 0.f);
+}
+
+__global__ void k_zero_unmeasured(float2* buf, const uint8_t* flags, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (!(flags[i] & kMeasured)) buf[i] = make_float2(0.f, 0.f);
+}
+
+// f1 of the difference map (reference src/phasing.c:1260-1274)
+__global__ void k_dm_f1(const float2* g, const uint8_t* flags, float gamma1, float2* out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float2 v = g[i];
+    out[i] = (flags[i] & kInsideSupport) ? v : make_float2(-gamma1 * v.x, -gamma1 * v.y);
+  }
+}
+
+// --- reductions: fixed grid, per-block partials, one-block final sum, so
+// --- the result does not depend on scheduling
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&acc)[NV], double* partials) {
+  __shared__ double red[NV * 32];
+  block_sum<NV, kRedThreads>(acc, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; ++q) partials[blockIdx.x * NV + q] = acc[q];
+}
+
+// E_R sums over the model before projection (reference src/phasing.c:826-832)
+__global__ __launch_bounds__(kRedThreads) void k_ereal(const float2* gp, const uint8_t* flags, long n, double* partials) {
+  double acc[2] = {0.0, 0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float2 v = gp[i];
+    const double m = (double)sqrtf(v.y * v.y + v.x * v.x);
+    if (!(flags[i] & kInsideSupport)) acc[0] += m * m;
+    acc[1] += m * m;
+  }
+  write_partials<2>(acc, partials);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sumsq_c64(const float2* in, long n, double* partials) {
+  double acc[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float2 v = in[i];
+    acc[0] += (double)(v.y * v.y + v.x * v.x);
+  }
+  write_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sumsq_f32(const float* in, long n, double* partials) {
+  double acc[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc[0] += (double)(in[i] * in[i]);
+  write_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_count_support(const uint8_t* flags, long n, double* partials) {
+  double acc[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc[0] += (flags[i] & kInsideSupport) ? 1.0 : 0.0;
+  write_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_blocks(const double* partials, int nblocks, int nv, double* out) {
+  __shared__ double sh[1024];
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(long)b * nv + q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
+    __syncthreads();
+  }
+}
+
+// --- exact k-th largest |blur| (descending index k, the reference's
+// --- qsort-then-index, src/support_update.c:143-144) by a three-level radix
+// --- select over the float bit patterns (non-negative floats order like
+// --- their bits): 11 + 11 + 10 bits.
+__global__ void k_ctl_init(unsigned* ctl, unsigned k) {
+  if (threadIdx.x == 0) {
+    ctl[kCtlK] = k;
+    ctl[kCtlPrefix] = 0;
+    ctl[kCtlThresh] = 0;
+    ctl[kCtlMax] = 0;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_select(const unsigned* hist, int nbins, unsigned* ctl, int level) {
+  __shared__ unsigned scan[1024];
+  const int per = nbins / 1024;  // 1 or 2 bins per thread, walked high to low
+  const int tid = threadIdx.x;
+  const unsigned k = ctl[kCtlK];
+  unsigned local = 0;
+  for (int q = 0; q < per; ++q) local += hist[nbins - 1 - (tid * per + q)];
+  scan[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned v = (tid >= off) ? scan[tid - off] : 0u;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  unsigned before = scan[tid] - local;  // elements in strictly higher bins
+  for (int q = 0; q < per; ++q) {
+    const int b = nbins - 1 - (tid * per + q);
+    const unsigned c = hist[b];
+    if (c > 0 && before <= k && k < before + c) {
+      ctl[kCtlK] = k - before;
+      if (level == 1) ctl[kCtlPrefix] = (unsigned)b;
+      else if (level == 2) ctl[kCtlPrefix] = (ctl[kCtlPrefix] << 11) | (unsigned)b;
+      else ctl[kCtlThresh] = (ctl[kCtlPrefix] << 10) | (unsigned)b;
+    }
+    before += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, unsigned* hist, const unsigned* ctl, int level) {
+  __shared__ unsigned h[2048];
+  const int nb = (level == 2) ? 2048 : 1024;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  const unsigned pre = ctl[kCtlPrefix];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const unsigned b = __float_as_uint(absb[i]);
+    if (level == 2) {
+      if ((b >> 21) == pre) atomicAdd(&h[(b >> 10) & 2047u], 1u);
+    } else {
+      if ((b >> 10) == pre) atomicAdd(&h[b & 1023u], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// support bit = |blur| > threshold (reference src/support_update.c:392-400);
+// mode 0: threshold = selected value, mode 1: max * rel (src/support_update.c:175)
+__global__ void k_mask(const float* absb, uint8_t* flags, long n, const unsigned* ctl, int mode, float rel) {
+  const float t = (mode == 0) ? __uint_as_float(ctl[kCtlThresh]) : __uint_as_float(ctl[kCtlMax]) * rel;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const uint8_t f = flags[i];
+    flags[i] = (absb[i] > t) ? (uint8_t)(f | kInsideSupport) : (uint8_t)(f & ~kInsideSupport);
+  }
+}
+
+inline dim3 ew_grid(long n) {
+  long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// shape helpers shared by the engine and the one-shot transforms
+// ---------------------------------------------------------------------------
+struct Shape {
+  int nx, ny, nz;
+  const LengthEntry *ex, *ey, *ez;
+  const float2 *twx, *twy, *twz;
+  LineGeom gy, gz;
+};
+
+static int make_shape(int nx, int ny, int nz, Shape* sh) {
+  memset(sh, 0, sizeof(*sh));
+  sh->nx = nx;
+  sh->ny = ny;
+  sh->nz = nz;
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("invalid shape %d x %d x %d", nx, ny, nz);
+    return -11;
+  }
+  if ((long)nx * ny * nz >= (1L << 31)) {
+    set_error("array of %ld elements exceeds the int range of the phasing API", (long)nx * ny * nz);
+    return -11;
+  }
+  const int dims[3] = {nx, ny, nz};
+  const LengthEntry** e[3] = {&sh->ex, &sh->ey, &sh->ez};
+  const float2** t[3] = {&sh->twx, &sh->twy, &sh->twz};
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] == 1) continue;
+    *e[a] = find_length(dims[a]);
+    if (!*e[a]) {
+      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
+      return -11;
+    }
+    *t[a] = twiddles(dims[a]);
+    if (!*t[a]) {
+      set_error("could not build twiddle table for length %d", dims[a]);
+      return -20;
+    }
+  }
+  sh->gy = LineGeom{nx, nz, (long)nx * ny, (long)nx};
+  sh->gz = LineGeom{nx * ny, 1, 0L, (long)nx * ny};
+  return 0;
+}
+
+// FFT over all non-trivial axes: x by a row pass, then y and z by column passes
+static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
+  const float2* src = in;
+  if (sh.ex) {
+    RowArgs r{};
+    r.in = in;
+    r.out = out;
+    r.tw = sh.twx;
+    r.nlines = sh.ny * sh.nz;
+    r.L = sh.nx;
+    sh.ex->row(dir < 0 ? kRowFwd : kRowInv, r, s);
+    src = out;
+  }
+  ColArgs c{};
+  if (sh.ey) {
+    c.in = src;
+    c.W = out;
+    c.tw = sh.twy;
+    c.geo = sh.gy;
+    sh.ey->col(dir < 0 ? kColFwd : kColInv, c, s);
+    src = out;
+  }
+  if (sh.ez) {
+    c.in = src;
+    c.W = out;
+    c.tw = sh.twz;
+    c.geo = sh.gz;
+    sh.ez->col(dir < 0 ? kColFwd : kColInv, c, s);
+    src = out;
+  }
+  if (src == in && in != out)
+    hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz, hipMemcpyDeviceToDevice, s);
+}
+
+// Column work after the x pass: the last non-trivial axis gets `mode`; the
+// middle axis (y of a 3-D array) is transformed forward before and inverse
+// after it.  For kColEFourier nothing is written back.
+static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2* out, ColArgs base, hipStream_t s) {
+  const bool three = sh.ez && sh.ey;
+  const LengthEntry* last = sh.ez ? sh.ez : sh.ey;
+  ColArgs a = base;
+  if (three) {
+    a.in = in;
+    a.W = out;
+    a.tw = sh.twy;
+    a.geo = sh.gy;
+    sh.ey->col(kColFwd, a, s);
+    a = base;
+    a.in = out;
+    a.W = out;
+    a.tw = sh.twz;
+    a.geo = sh.gz;
+    sh.ez->col(mode, a, s);
+    if (mode != kColEFourier) {
+      a = base;
+      a.in = out;
+      a.W = out;
+      a.tw = sh.twy;
+      a.geo = sh.gy;
+      sh.ey->col(kColInv, a, s);
+    }
+  } else {
+    a.in = in;
+    a.W = out;
+    a.tw = sh.ez ? sh.twz : sh.twy;
+    a.geo = sh.ez ? sh.gz : sh.gy;
+    last->col(mode, a, s);
+  }
+}
+
+static int last_axis_blocks(const Shape& sh) {
+  const LengthEntry* last = sh.ez ? sh.ez : sh.ey;
+  return last->col_blocks(sh.ez ? sh.gz : sh.gy);
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+Engine* Engine::create(int nx, int ny, int nz) {
+  if (!device_available()) {
+    set_error("no HIP device available");
+    return nullptr;
+  }
+  Shape sh;
+  if (make_shape(nx, ny, nz, &sh)) return nullptr;
+  if (!sh.ex || (!sh.ey && !sh.ez)) {
+    set_error("the HIP engine phases 2-D and 3-D arrays with nx > 1 (got %d x %d x %d)", nx, ny, nz);
+    return nullptr;
+  }
+  Engine* e = new Engine();
+  e->nx = nx;
+  e->ny = ny;
+  e->nz = nz;
+  e->ndim = (nz > 1) ? 3 : 2;
+  e->N = (long)nx * ny * nz;
+  e->ex = sh.ex;
+  e->ey = sh.ey;
+  e->ez = sh.ez;
+  e->twx = sh.twx;
+  e->twy = sh.twy;
+  e->twz = sh.twz;
+  e->gy = sh.gy;
+  e->gz = sh.gz;
+  if (e->alloc_all()) {
+    delete e;
+    return nullptr;
+  }
+  return e;
+}
+
+int Engine::alloc_all() {
+  SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  SPX_HIP(hipEventCreate(&ev0));
+  SPX_HIP(hipEventCreate(&ev1));
+  const size_t c = sizeof(float2) * (size_t)N, f = sizeof(float) * (size_t)N;
+  SPX_HIP(hipMalloc(&g[0], c));
+  SPX_HIP(hipMalloc(&g[1], c));
+  SPX_HIP(hipMalloc(&W, c));
+  SPX_HIP(hipMalloc(&gp, c));
+  SPX_HIP(hipMalloc(&S1, c));
+  SPX_HIP(hipMalloc(&ampx, f));
+  SPX_HIP(hipMalloc(&absb, f));
+  SPX_HIP(hipMalloc(&flags, (size_t)N));
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  max_partials = last_axis_blocks(sh);
+  if (max_partials < kRedBlocks) max_partials = kRedBlocks;
+  SPX_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)max_partials));
+  SPX_HIP(hipMalloc(&red, sizeof(double) * 8));
+  SPX_HIP(hipMalloc(&hist, sizeof(unsigned) * 3 * 2048));
+  SPX_HIP(hipMalloc(&ctl, sizeof(unsigned) * kCtlWords));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[0], N, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[1], N, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, gp, N, make_float2(0.f, 0.f));
+  SPX_HIP(hipMemsetAsync(flags, 0, (size_t)N, s));
+  SPX_HIP(hipMemsetAsync(absb, 0, f, s));
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(N), dim3(256), 0, s, absb, flags, ampx, N);
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+Engine::~Engine() {
+  if (s) hipStreamSynchronize(s);
+  void* ptrs[] = {g[0], g[1], W, Wf, gp, S1, ampx, amp_min, amp_max, absb, phased, flags, partials, red, hist, ctl};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (ev0) hipEventDestroy(ev0);
+  if (ev1) hipEventDestroy(ev1);
+  if (s) hipStreamDestroy(s);
+}
+
+// encoded amplitudes: the raw values pass through `absb` (scratch) once
+int Engine::upload_amplitudes(const float* amp, const uint8_t* fl) {
+  SPX_HIP(hipMemcpyAsync(absb, amp, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(N), dim3(256), 0, s, absb, flags, ampx, N);
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipStreamSynchronize(s));
+  ++flags_version;
+  return 0;
+}
+
+int Engine::upload_flags(const uint8_t* fl) {
+  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  ++flags_version;
+  return 0;
+}
+
+int Engine::download_flags(uint8_t* fl) {
+  SPX_HIP(hipMemcpyAsync(fl, flags, (size_t)N, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::upload_margins(const float* lo, const float* hi) {
+  if (!amp_min) SPX_HIP(hipMalloc(&amp_min, sizeof(float) * N));
+  if (!amp_max) SPX_HIP(hipMalloc(&amp_max, sizeof(float) * N));
+  SPX_HIP(hipMemcpyAsync(amp_min, lo, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(amp_max, hi, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  have_margins = true;
+  return 0;
+}
+
+int Engine::upload_phased(const float2* ph, const uint8_t* fl) {
+  if (!phased) SPX_HIP(hipMalloc(&phased, sizeof(float2) * N));
+  SPX_HIP(hipMemcpyAsync(phased, ph, sizeof(float2) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  have_phased = true;
+  ++flags_version;
+  return 0;
+}
+
+int Engine::upload_model(const float2* model) {
+  SPX_HIP(hipMemcpyAsync(g[cur], model, sizeof(float2) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  invalidate_R();
+  return 0;
+}
+
+int Engine::download(const float2* dev, float2* host) {
+  SPX_HIP(hipMemcpyAsync(host, dev, sizeof(float2) * N, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::fft_nd(const float2* in, float2* out, int dir) {
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  enqueue_fft(sh, in, out, dir, s);
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
+int Engine::reduce_partials(int nparts, int nv, double* host_out) {
+  hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(1024), 0, s, partials, nparts, nv, red);
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipMemcpyAsync(host_out, red, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd) {
+  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  const float inv_n = (float)(1.0 / (double)N);
+  if (fl & 1 /* SpModelRandomPhases */ || fl & 2 /* SpModelZeroPhases */) {
+    hipLaunchKernelGGL(k_init_phases, ew_grid(N), dim3(256), 0, s, absb, S1, N, sd, (fl & 1) ? 1 : 0);
+    if (fft_nd(S1, S1, +1)) return -20;
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, S1, g[cur], N, inv_n);
+  } else if (fl & 4 /* SpModelRandomValues */) {
+    // reference src/phasing.c:666-679: scale so that sum |g|^2 = sum amp^2
+    double sums[2];
+    hipLaunchKernelGGL(k_sumsq_f32, dim3(kRedBlocks), dim3(kRedThreads), 0, s, absb, N, partials);
+    if (reduce_partials(kRedBlocks, 1, &sums[0])) return -20;
+    hipLaunchKernelGGL(k_random_values, ew_grid(N), dim3(256), 0, s, g[cur], N, sd);
+    hipLaunchKernelGGL(k_sumsq_c64, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g[cur], N, partials);
+    if (reduce_partials(kRedBlocks, 1, &sums[1])) return -20;
+    const double mean = sums[0] / (double)N;
+    const float sc = (float)sqrt(mean / sums[1]);
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, g[cur], g[cur], N, sc);
+  } else {
+    return -3;
+  }
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
+int Engine::init_model_user(const float2* model, int) {
+  SPX_HIP(hipMemcpyAsync(g[cur], model, sizeof(float2) * N, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+int Engine::finish_model_init(int fl) {
+  if (fl & 256 /* SpModelMaskedOutZeroed */) {
+    if (fft_nd(g[cur], S1, -1)) return -20;
+    hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(N), dim3(256), 0, s, S1, flags, N);
+    if (fft_nd(S1, S1, +1)) return -20;
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, S1, g[cur], N, (float)(1.0 / (double)N));
+  }
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[cur ^ 1], N, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, gp, N, make_float2(0.f, 0.f));
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipStreamSynchronize(s));
+  invalidate_R();
+  return 0;
+}
+
+int Engine::ensure_R() {
+  if (R_valid) return 0;
+  RowArgs r{};
+  r.in = g[cur];
+  r.out = W;
+  r.tw = twx;
+  r.nlines = ny * nz;
+  r.L = nx;
+  ex->row(kRowFwd, r, s);
+  SPX_HIP(hipGetLastError());
+  R_valid = true;
+  return 0;
+}
+
+int Engine::ensure_Rf(float gamma1) {
+  if (!Wf) SPX_HIP(hipMalloc(&Wf, sizeof(float2) * N));
+  if (Rf_valid && Rf_version == flags_version && Rf_gamma1 == gamma1) return 0;
+  hipLaunchKernelGGL(k_dm_f1, ew_grid(N), dim3(256), 0, s, g[cur], flags, gamma1, Wf, N);
+  RowArgs r{};
+  r.in = Wf;
+  r.out = Wf;
+  r.tw = twx;
+  r.nlines = ny * nz;
+  r.L = nx;
+  ex->row(kRowFwd, r, s);
+  SPX_HIP(hipGetLastError());
+  Rf_valid = true;
+  Rf_version = flags_version;
+  Rf_gamma1 = gamma1;
+  return 0;
+}
+
+int Engine::iterate(const IterParams& p, const float* betas, int n, bool want_gp) {
+  if (n <= 0) return 0;
+  const bool dm = (p.alg == kDiffMap);
+  if (int r = ensure_R()) return r;
+  if (dm)
+    if (int r = ensure_Rf(p.gamma1)) return r;
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  ColArgs ca{};
+  ca.ampx = ampx;
+  ca.amp_min = amp_min;
+  ca.amp_max = amp_max;
+  ca.phased_amp = phased;
+  ca.flags = flags;
+  ca.centro = (p.constraints & kCentrosymmetricObject) ? 1 : 0;
+  ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
+  ca.objective = dm ? 0 : p.objective;
+  ColArgs caf = ca;
+  caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
+  UpdateArgs u{};
+  u.W = W;
+  u.Wf = Wf;
+  u.flags = flags;
+  u.tw = twx;
+  u.nlines = ny * nz;
+  u.alg = p.alg;
+  u.constraints = p.constraints;
+  u.gamma1 = p.gamma1;
+  u.gamma2 = p.gamma2;
+  u.scale = (float)(1.0 / (double)N);
+  u.nsize = (float)N;
+  for (int i = 0; i < n; ++i) {
+    enqueue_fourier(sh, kColProject, W, W, ca, s);
+    if (dm) enqueue_fourier(sh, kColProject, Wf, Wf, caf, s);
+    u.gcur = g[cur];
+    u.gnew = g[cur ^ 1];
+    u.gp = (want_gp && i == n - 1) ? gp : nullptr;
+    u.beta = betas[i];
+    ex->update(u, dm, s);
+    cur ^= 1;
+  }
+  SPX_HIP(hipGetLastError());
+  if (dm) {
+    Rf_version = flags_version;
+    Rf_gamma1 = p.gamma1;
+  } else {
+    Rf_valid = false;
+  }
+  return 0;
+}
+
+int Engine::time_iterations(const IterParams& p, const float* betas, int n, float* ms) {
+  if (int r = ensure_R()) return r;
+  if (p.alg == kDiffMap)
+    if (int r = ensure_Rf(p.gamma1)) return r;
+  SPX_HIP(hipEventRecord(ev0, s));
+  if (int r = iterate(p, betas, n, false)) return r;
+  SPX_HIP(hipEventRecord(ev1, s));
+  SPX_HIP(hipEventSynchronize(ev1));
+  SPX_HIP(hipEventElapsedTime(ms, ev0, ev1));
+  return 0;
+}
+
+int Engine::efourier(double out[4], float2* fout) {
+  if (int r = ensure_R()) return r;
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  ColArgs ca{};
+  ca.ampx = ampx;
+  ca.partials = partials;
+  ca.fout = fout;
+  // the 3-D path needs somewhere to put the y-transformed lines: S1
+  enqueue_fourier(sh, kColEFourier, W, S1, ca, s);
+  SPX_HIP(hipGetLastError());
+  return reduce_partials(last_axis_blocks(sh), 4, out);
+}
+
+int Engine::fmodel(float2* fout) {
+  double dummy[4];
+  return efourier(dummy, fout);
+}
+
+int Engine::ereal(double out[2]) {
+  hipLaunchKernelGGL(k_ereal, dim3(kRedBlocks), dim3(kRedThreads), 0, s, gp, flags, N, partials);
+  SPX_HIP(hipGetLastError());
+  return reduce_partials(kRedBlocks, 2, out);
+}
+
+int Engine::support_count(double* out) {
+  hipLaunchKernelGGL(k_count_support, dim3(kRedBlocks), dim3(kRedThreads), 0, s, flags, N, partials);
+  SPX_HIP(hipGetLastError());
+  return reduce_partials(kRedBlocks, 1, out);
+}
+
+int Engine::model_change(float2* out) {
+  hipLaunchKernelGGL(k_sub, ew_grid(N), dim3(256), 0, s, g[cur], g[cur ^ 1], out, N);
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
+int Engine::blur(const float2* in, float2* out, float sigma) {
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  RowArgs r{};
+  r.in = in;
+  r.out = out;
+  r.tw = twx;
+  r.nlines = ny * nz;
+  r.L = nx;
+  ex->row(kRowFwd, r, s);
+  ColArgs cb{};
+  cb.nx = nx;
+  cb.ny = ny;
+  cb.nz = nz;
+  cb.sigma = sigma;
+  enqueue_fourier(sh, kColBlur, out, out, cb, s);
+  r.in = out;
+  ex->row(kRowInv, r, s);
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
+// |blur(|g1|, sigma)| into absb, with the level-1 histogram and the running
+// maximum (reference src/support_update.c:138-141,169-172)
+int Engine::blur_to_abs(float sigma, unsigned* h, unsigned* maxbits) {
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  RowArgs r{};
+  r.in = g[cur];
+  r.out = S1;
+  r.tw = twx;
+  r.nlines = ny * nz;
+  r.L = nx;
+  ex->row(kRowAbsFwd, r, s);
+  ColArgs cb{};
+  cb.nx = nx;
+  cb.ny = ny;
+  cb.nz = nz;
+  cb.sigma = sigma;
+  enqueue_fourier(sh, kColBlur, S1, S1, cb, s);
+  r.in = S1;
+  r.out = nullptr;
+  r.absb = absb;
+  r.hist = h;
+  r.maxbits = maxbits;
+  ex->row(kRowBlurOut, r, s);
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
+int Engine::shrinkwrap_area(float sigma, float area) {
+  // k = (int)(N * area) with N converted to float first (src/support_update.c:144)
+  long k = (long)(int)((float)N * area);
+  if (k >= N) k = N - 1;
+  if (k < 0) k = 0;
+  hipLaunchKernelGGL(k_fill_u32, dim3(24), dim3(256), 0, s, hist, 3L * 2048, 0u);
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, (unsigned)k);
+  if (int rc = blur_to_abs(sigma, hist, ctl + kCtlMax)) return rc;
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist, 2048, ctl, 1);
+  hipLaunchKernelGGL(k_hist_level, ew_grid(N), dim3(256), 0, s, absb, N, hist + 2048, ctl, 2);
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist + 2048, 2048, ctl, 2);
+  hipLaunchKernelGGL(k_hist_level, ew_grid(N), dim3(256), 0, s, absb, N, hist + 4096, ctl, 3);
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist + 4096, 1024, ctl, 3);
+  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 0, 0.f);
+  SPX_HIP(hipGetLastError());
+  ++flags_version;
+  return 0;
+}
+
+int Engine::shrinkwrap_threshold(float sigma, float rel) {
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, 0u);
+  if (int rc = blur_to_abs(sigma, nullptr, ctl + kCtlMax)) return rc;
+  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 1, rel);
+  SPX_HIP(hipGetLastError());
+  ++flags_version;
+  return 0;
+}
+
+// support from the thresholded Patterson map |IFFT(amp^2)| > max * value
+// (reference src/phasing.c:744-763)
+int Engine::patterson_support(const float* amp_raw, float value) {
+  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_amp2, ew_grid(N), dim3(256), 0, s, absb, S1, N);
+  ColArgs c{};
+  if (ey) {
+    c.in = S1;
+    c.W = S1;
+    c.tw = twy;
+    c.geo = gy;
+    ey->col(kColInv, c, s);
+  }
+  if (ez) {
+    c.in = S1;
+    c.W = S1;
+    c.tw = twz;
+    c.geo = gz;
+    ez->col(kColInv, c, s);
+  }
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, 0u);
+  RowArgs r{};
+  r.in = S1;
+  r.absb = absb;
+  r.maxbits = ctl + kCtlMax;
+  r.tw = twx;
+  r.nlines = ny * nz;
+  r.L = nx;
+  ex->row(kRowBlurOut, r, s);
+  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 1, value);
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipStreamSynchronize(s));
+  ++flags_version;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// one-shot transforms for the Image-level API
+// ---------------------------------------------------------------------------
+static std::mutex g_oneshot_mu;
+
+static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int dir, float sigma, bool is_blur) {
+  std::lock_guard<std::mutex> lock(g_oneshot_mu);
+  if (!device_available()) {
+    set_error("no HIP device available");
+    return -10;
+  }
+  Shape sh;
+  if (int rc = make_shape(nx, ny, nz, &sh)) return rc;
+  static hipStream_t s = nullptr;
+  if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const size_t bytes = sizeof(float2) * (size_t)nx * ny * nz;
+  float2* d = nullptr;
+  SPX_HIP(hipMalloc(&d, bytes));
+  int rc = 0;
+  if (hipMemcpyAsync(d, in, bytes, hipMemcpyHostToDevice, s) != hipSuccess) rc = -20;
+  if (!rc) {
+    if (!is_blur) {
+      enqueue_fft(sh, d, d, dir, s);
+    } else {
+      if (!sh.ex) {
+        set_error("blur needs nx > 1");
+        rc = -11;
+      } else {
+        RowArgs r{};
+        r.in = d;
+        r.out = d;
+        r.tw = sh.twx;
+        r.nlines = ny * nz;
+        r.L = nx;
+        sh.ex->row(kRowFwd, r, s);
+        if (sh.ey || sh.ez) {
+          ColArgs cb{};
+          cb.nx = nx;
+          cb.ny = ny;
+          cb.nz = nz;
+          cb.sigma = sigma;
+          enqueue_fourier(sh, kColBlur, d, d, cb, s);
+        } else {
+          rc = -11;
+          set_error("blur needs a 2-D or 3-D array");
+        }
+        sh.ex->row(kRowInv, r, s);
+      }
+    }
+  }
+  if (!rc && hipGetLastError() != hipSuccess) rc = -20;
+  if (!rc && hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -20;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = -20;
+  hipFree(d);
+  if (rc == -20) set_error("HIP error in one-shot transform");
+  return rc;
+}
+
+int fft_host(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
+  return oneshot(in, out, nx, ny, nz, dir, 0.f, false);
+}
+
+int blur_host(const float2* in, float2* out, int nx, int ny, int nz, float sigma) {
+  return oneshot(in, out, nx, ny, nz, 0, sigma, true);
+}
+
+int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
+  std::lock_guard<std::mutex> lock(g_oneshot_mu);
+  if (!device_available()) {
+    set_error("no HIP device available");
+    return -10;
+  }
+  Shape sh;
+  if (int rc = make_shape(nx, ny, nz, &sh)) return rc;
+  static hipStream_t s = nullptr;
+  if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  enqueue_fft(sh, in, out, dir, s);
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace spx

@@ -1,0 +1,508 @@
+// Pass kernels of the HIP phasing engine.
+//
+// Every array is (nz, ny, nx) row-major with x fastest, like the reference's
+// sp_c3matrix (include/spimage/linear_alg.h:436-438).  One work-group owns
+// whole lines along one axis, reads them once into LDS, transforms them there
+// and writes them back, so every pass may run in place.
+//
+//   row passes    lines along x (contiguous), M lines per work-group
+//   column passes lines along y or z (strided), C adjacent lines per
+//                 work-group so that every global access moves C*8
+//                 contiguous bytes
+//
+// The Fourier-space work of an iteration (constraints + modulus projection,
+// reference src/phasing.c:1061-1109) runs in the column pass of the last axis
+// between its forward and inverse transforms; the real-space update (ER/HIO/
+// RAAR/DM, src/phasing.c:1111-1258), the 1/N scale and the support
+// constraints (src/phasing.c:1026-1059) run in the row pass between the
+// inverse x-transform of this iteration and the forward x-transform of the
+// next one.
+#pragma once
+#include "fft_core.h"
+
+namespace spx {
+
+// pixel flag bits, same values as SpPhasingPixelFlags (reference
+// include/spimage/phasing.h:19)
+constexpr uint8_t kInsideSupport = 1;
+constexpr uint8_t kMeasured = 2;
+
+// Device amplitude encoding: unmeasured pixels carry this NaN pattern so the
+// projection needs no separate flag read.  A measured NaN amplitude is stored
+// as the canonical quiet NaN, whose pattern differs.
+constexpr uint32_t kUnmeasuredBits = 0xFFFFFFFFu;
+
+// Phasing algorithm ids, same values as SpPhasingAlgorithmType
+// (reference include/spimage/phasing.h:16).
+enum : int { kHIO = 1, kRAAR = 2, kDiffMap = 3, kER = 4 };
+// SpPhasingConstraints bits (reference include/spimage/phasing.h:17)
+enum : int {
+  kRealObject = 1, kPositiveRealObject = 2, kPositiveComplexObject = 4,
+  kPositivityFlipping = 8, kCentrosymmetricObject = 16, kAmplitudeErrorMargin = 32
+};
+
+// Map hardware work-group order (round-robin over the 8 XCDs) onto tiles so
+// that each XCD receives a contiguous range: neighbouring column groups share
+// 128-byte lines and then share one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, k = bid >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------------------
+// Argument blocks (passed by value).
+// ---------------------------------------------------------------------------
+struct LineGeom {
+  // line element i of line (outer, c) lives at outer*outer_stride + c + i*axis_stride
+  int ncontig;       // lines adjacent in memory (x extent for column passes)
+  int nouter;        // number of outer indices
+  long outer_stride;
+  long axis_stride;
+};
+
+enum RowMode : int {
+  kRowFwd = 0,        // out = FFT_x(in)
+  kRowInv = 1,        // out = IFFT_x(in)
+  kRowAbsFwd = 2,     // out = FFT_x(|in|)         (blur input, src/support_update.c:139-141)
+  kRowBlurOut = 3,    // absb = |IFFT_x(in)| plus max / level-1 histogram
+};
+
+struct RowArgs {
+  const float2* in;
+  float2* out;
+  float* absb;             // kRowBlurOut: |blur| output (f32)
+  unsigned* hist;          // kRowBlurOut: level-1 histogram (2048 bins), or null
+  unsigned* maxbits;       // kRowBlurOut: running max of |blur| as bits, or null
+  const float2* tw;
+  int nlines;
+  int L;
+};
+
+struct UpdateArgs {
+  float2* W;               // in: column-pass output C; out: FFT_x of the new iterate
+  float2* Wf;              // diff map: same for f1
+  const float2* gcur;      // iterate g_k (the reference's g0 in this iteration)
+  float2* gnew;            // iterate g_{k+1}
+  float2* gp;              // model before projection, or null
+  const uint8_t* flags;
+  const float2* tw;
+  int nlines;
+  int alg;
+  int constraints;
+  float beta;
+  float gamma1, gamma2;
+  float scale;             // 1/N as float
+  float nsize;             // N as float (diff map divides by it)
+};
+
+enum ColMode : int {
+  kColFwd = 0,            // in place FFT along the axis
+  kColInv = 1,            // in place IFFT along the axis
+  kColProject = 2,        // FFT, Fourier constraints + projection, IFFT
+  kColEFourier = 3,       // FFT, error-metric partial sums, optional write
+  kColBlur = 4,           // FFT, Gaussian transfer function, IFFT
+};
+
+struct ColArgs {
+  const float2* in;          // lines are read from here ...
+  float2* W;                 // ... and written here (may alias `in`)
+  const float2* tw;
+  LineGeom geo;
+  // projection
+  const float* ampx;         // encoded amplitudes (kUnmeasuredBits for unmeasured)
+  const float* amp_min;
+  const float* amp_max;
+  const float2* phased_amp;  // SpRecoverAmplitudes objective
+  const uint8_t* flags;      // read only by the phased-amplitude objective
+  int centro;                // apply the centrosymmetric Fourier constraint
+  int margin;                // SpAmplitudeErrorMargin
+  int objective;             // 0 phases, 1 amplitudes
+  // efourier
+  double* partials;          // 4 doubles per work-group
+  float2* fout;              // optional copy of the transform (fmodel)
+  // blur
+  int nx, ny, nz;
+  float sigma;
+};
+
+// ---------------------------------------------------------------------------
+// Shared pieces.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float cabsf_ref(float2 a) {
+  // sp_cabs: sqrt(im*im + re*re) evaluated in float (linear_alg.h:163-165)
+  return sqrtf(a.y * a.y + a.x * a.x);
+}
+
+// Modulus projection of one Fourier-space pixel (src/phasing.c:1070-1100).
+__device__ __forceinline__ float2 project_pixel(float2 g, float ampbits_f, const ColArgs& a, long idx) {
+  const uint32_t bits = __float_as_uint(ampbits_f);
+  if (bits == kUnmeasuredBits) return g;
+  const float amp = ampbits_f;
+  float m = 1.f;
+  if (!a.margin) {
+    m = amp / cabsf_ref(g);
+  } else {
+    const float aa = cabsf_ref(g);
+    const float lo = a.amp_min[idx], hi = a.amp_max[idx];
+    if (aa < lo) m = lo / aa;
+    else if (aa > hi) m = hi / aa;
+  }
+  if (isfinite(m)) return make_float2(g.x * m, g.y * m);
+  return make_float2(amp, 0.f);
+}
+
+// Real-space support constraints (src/phasing.c:1026-1059), inside support only.
+__device__ __forceinline__ float2 apply_constraints(float2 g, int c) {
+  if (c & kRealObject) {
+    g.y = 0.f;
+  } else if (c & kPositiveRealObject) {
+    if (g.x < 0.f) g.x = (c & kPositivityFlipping) ? fabsf(g.x) : 0.f;
+    g.y = 0.f;
+  } else if (c & kPositiveComplexObject) {
+    if (g.x < 0.f) g.x = (c & kPositivityFlipping) ? fabsf(g.x) : 0.f;
+    if (g.y < 0.f) g.y = (c & kPositivityFlipping) ? fabsf(g.y) : 0.f;
+  }
+  return g;
+}
+
+// Block-wide sum of NV doubles; result valid in thread 0.
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV*32 */) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[q] = x;
+  }
+  constexpr int NW = (NT + 63) / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) scratch[q * 32 + w] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int i = 0; i < NW; ++i) s += scratch[q * 32 + i];
+      v[q] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row kernels.
+// ---------------------------------------------------------------------------
+template <int L>
+struct RowShape {
+  static constexpr int T = PlanOf<L>::T;
+  static constexpr int M = (T >= 256) ? 1 : (256 / T);
+  static constexpr int NT = T * M;
+};
+
+template <int L, int MODE>
+__global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
+  constexpr int T = RowShape<L>::T, M = RowShape<L>::M, NT = RowShape<L>::NT;
+  using View = LdsView<L, M, false>;
+  __shared__ float2 sh[View::SIZE];
+  __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
+  const int tid = threadIdx.x;
+  const int g = tid / T, t = tid - g * T;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int line = blk * M + g;
+  const bool act = line < a.nlines;
+  const View v{sh, g};
+  const long base = (long)line * L;
+
+  if constexpr (MODE == kRowBlurOut) {
+    if (a.hist)
+      for (int i = tid; i < 2048; i += NT) hsh[i] = 0u;
+  }
+#pragma unroll 4
+  for (int pos = t; pos < L; pos += T) {
+    float2 x = make_float2(0.f, 0.f);
+    if (act) {
+      x = a.in[base + pos];
+      if constexpr (MODE == kRowAbsFwd) x = make_float2(cabsf_ref(x), 0.f);
+    }
+    v.at(pos) = x;
+  }
+  __syncthreads();
+  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, +1>(v, t, a.tw);
+  else fft_line<L, -1>(v, t, a.tw);
+
+  if constexpr (MODE == kRowBlurOut) {
+    float mx = 0.f;
+    for (int pos = t; pos < L; pos += T) {
+      if (act) {
+        const float m = cabsf_ref(v.at(pos));
+        a.absb[base + pos] = m;
+        mx = fmaxf(mx, m);
+        if (a.hist) atomicAdd(&hsh[__float_as_uint(m) >> 21], 1u);
+      }
+    }
+    if (a.maxbits) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+      if ((tid & 63) == 0) atomicMax(a.maxbits, __float_as_uint(mx));
+    }
+    if (a.hist) {
+      __syncthreads();
+      for (int i = tid; i < 2048; i += NT)
+        if (hsh[i]) atomicAdd(&a.hist[i], hsh[i]);
+    }
+  } else {
+    for (int pos = t; pos < L; pos += T)
+      if (act) a.out[base + pos] = v.at(pos);
+  }
+}
+
+// The fused real-space pass of one iteration: inverse x-transform of the
+// column-pass output, 1/N scale, algorithm update, support constraints, store
+// of the new iterate, then forward x-transform of the new iterate (and, for
+// the difference map, of its f1) for the next iteration's column pass.
+template <int L, bool DM>
+__global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
+  constexpr int T = RowShape<L>::T, M = RowShape<L>::M;
+  using View = LdsView<L, M, false>;
+  __shared__ float2 sh[View::SIZE];
+  __shared__ float2 shf[DM ? View::SIZE : 1];
+  const int tid = threadIdx.x;
+  const int g = tid / T, t = tid - g * T;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int line = blk * M + g;
+  const bool act = line < a.nlines;
+  const View v{sh, g};
+  const View vf{shf, g};
+  const long base = (long)line * L;
+
+#pragma unroll 4
+  for (int pos = t; pos < L; pos += T) {
+    v.at(pos) = act ? a.W[base + pos] : make_float2(0.f, 0.f);
+    if constexpr (DM) vf.at(pos) = act ? a.Wf[base + pos] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_line<L, +1>(v, t, a.tw);
+  if constexpr (DM) fft_line<L, +1>(vf, t, a.tw);
+
+  for (int pos = t; pos < L; pos += T) {
+    if (!act) continue;
+    const long idx = base + pos;
+    const float2 P = v.at(pos);
+    const float2 g0 = a.gcur[idx];
+    const bool in = a.flags[idx] & kInsideSupport;
+    float2 g1;
+    if constexpr (!DM) {
+      const float2 gp = make_float2(P.x * a.scale, P.y * a.scale);
+      if (a.gp) a.gp[idx] = gp;
+      if (in) {
+        g1 = gp;
+      } else if (a.alg == kHIO) {
+        g1 = make_float2(g0.x - gp.x * a.beta, g0.y - gp.y * a.beta);
+      } else if (a.alg == kRAAR) {
+        const float c1 = 1.f - 2.f * a.beta;
+        g1 = make_float2(gp.x * c1 + g0.x * a.beta, gp.y * c1 + g0.y * a.beta);
+      } else {
+        g1 = make_float2(0.f, 0.f);
+      }
+    } else {
+      // src/phasing.c:1243-1252; gp stays unscaled like the CPU engine
+      if (a.gp) a.gp[idx] = P;
+      const float2 Pf = vf.at(pos);
+      if (in) {
+        g1.x = g0.x + a.beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x);
+        g1.y = g0.y + a.beta * ((1.f + a.gamma2) * P.y / a.nsize - a.gamma2 * g0.y);
+      } else {
+        g1 = g0;
+      }
+      g1.x -= a.beta * Pf.x / a.nsize;
+      g1.y -= a.beta * Pf.y / a.nsize;
+    }
+    if (in) g1 = apply_constraints(g1, a.constraints);
+    a.gnew[idx] = g1;
+    v.at(pos) = g1;
+    if constexpr (DM) {
+      // f1 of the next iteration (src/phasing.c:1260-1274)
+      vf.at(pos) = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
+    }
+  }
+  __syncthreads();
+  fft_line<L, -1>(v, t, a.tw);
+  if constexpr (DM) fft_line<L, -1>(vf, t, a.tw);
+  for (int pos = t; pos < L; pos += T) {
+    if (!act) continue;
+    a.W[base + pos] = v.at(pos);
+    if constexpr (DM) a.Wf[base + pos] = vf.at(pos);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column kernels.
+// ---------------------------------------------------------------------------
+template <int L>
+struct ColShape {
+  static constexpr int T = PlanOf<L>::T;
+  // adjacent lines per work-group: 8 (64 contiguous bytes) for long lines,
+  // more for short ones so that a work-group still has >= 128 threads
+  // (at most 64 KiB of LDS: 8192 complex values)
+  static constexpr int Cmem = (8192 / L) >= 1 ? (8192 / L) : 1;
+  static constexpr int C0 = ((L >= 256) ? 8 : 16) < Cmem ? ((L >= 256) ? 8 : 16) : Cmem;
+  static constexpr int C = (T * C0 >= 128 || L * C0 * 2 > 8192) ? C0 : ((128 / T) < 64 ? (128 / T) : 64);
+  static constexpr int NT = T * C;
+};
+
+template <int L, int MODE>
+__global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
+  constexpr int T = ColShape<L>::T, C = ColShape<L>::C, NT = ColShape<L>::NT;
+  using View = LdsView<L, C, true>;
+  __shared__ float2 sh[View::SIZE];
+  __shared__ double red[(MODE == kColEFourier) ? 4 * 32 : 1];
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int ngrp = (a.geo.ncontig + C - 1) / C;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int outer = blk / ngrp;
+  const int x = (blk - outer * ngrp) * C + c;
+  const bool act = x < a.geo.ncontig;
+  const long base = (long)outer * a.geo.outer_stride + x;
+  const long st = a.geo.axis_stride;
+  const View v{sh, c};
+
+#pragma unroll 4
+  for (int pos = t; pos < L; pos += T) v.at(pos) = act ? a.in[base + pos * st] : make_float2(0.f, 0.f);
+  __syncthreads();
+  if constexpr (MODE == kColInv) fft_line<L, +1>(v, t, a.tw);
+  else fft_line<L, -1>(v, t, a.tw);
+
+  if constexpr (MODE == kColProject) {
+    for (int pos = t; pos < L; pos += T) {
+      if (!act) continue;
+      const long idx = base + pos * st;
+      float2 G = v.at(pos);
+      if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
+      if (a.objective == 0) {
+        G = project_pixel(G, a.ampx[idx], a, idx);
+      } else if (a.flags[idx] & kMeasured) {
+        G = a.phased_amp[idx];  // src/phasing.c:1103-1109
+      }
+      v.at(pos) = G;
+    }
+    __syncthreads();
+    fft_line<L, +1>(v, t, a.tw);
+  } else if constexpr (MODE == kColBlur) {
+    // Gaussian transfer function of sp_gaussian_blur (src/image_filter.c:80-91)
+    // with dist_to_top_left (src/image_util.c:1012-1032), computed in double
+    // and rounded to float like the reference.
+    const double NN = (double)((float)((long)a.nx * a.ny * a.nz));
+    const double k0 = -2.0 * 9.8696044010893586188;  // -2 * pi^2
+    const double s2 = (double)a.sigma * (double)a.sigma;
+    for (int pos = t; pos < L; pos += T) {
+      if (!act) continue;
+      const long idx = base + pos * st;
+      const int ix = (int)(idx % a.nx);
+      const long rest = idx / a.nx;
+      const int iy = (int)(rest % a.ny);
+      const int iz = (int)(rest / a.ny);
+      const float dx = (a.nx - ix < ix) ? (float)(a.nx - ix) : (float)ix;
+      const float dy = (a.ny - iy < iy) ? (float)(a.ny - iy) : (float)iy;
+      const float dz = (a.nz - iz < iz) ? (float)(a.nz - iz) : (float)iz;
+      const float dist = (float)sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+      const float r = dist / (float)a.nx;
+      const float h = (float)(exp(k0 * ((double)r * (double)r) * s2) / NN);
+      const float2 F = v.at(pos);
+      v.at(pos) = make_float2(F.x * h, F.y * h);
+    }
+    __syncthreads();
+    fft_line<L, +1>(v, t, a.tw);
+  } else if constexpr (MODE == kColEFourier) {
+    // src/phasing.c:879-888: measured pixels add (|F|-|amp|)^2 and amp^2,
+    // unmeasured ones |F|^2; FcFo sums |amp| and |F| over measured pixels.
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // nom, den, fcfo_num, fcfo_den
+    for (int pos = t; pos < L; pos += T) {
+      if (!act) continue;
+      const long idx = base + pos * st;
+      const float2 F = v.at(pos);
+      if (a.fout) a.fout[idx] = F;
+      const float af = cabsf_ref(F);
+      const float ab = a.ampx[idx];
+      if (__float_as_uint(ab) != kUnmeasuredBits) {
+        const double aa = fabs((double)ab);
+        const double d = (double)af - aa;
+        acc[0] += d * d;
+        acc[1] += aa * aa;
+        acc[2] += aa;
+        acc[3] += (double)af;
+      } else {
+        acc[1] += (double)af * (double)af;
+      }
+    }
+    block_sum<4, NT>(acc, red);
+    if (tid == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a.partials[(long)blockIdx.x * 4 + q] = acc[q];
+    }
+    return;
+  }
+
+  for (int pos = t; pos < L; pos += T)
+    if (act) a.W[base + pos * st] = v.at(pos);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers (one instantiation per supported length).
+// ---------------------------------------------------------------------------
+template <int L>
+struct LengthOps {
+  static void row(int mode, const RowArgs& a, hipStream_t s) {
+    const dim3 grid((a.nlines + RowShape<L>::M - 1) / RowShape<L>::M), blk(RowShape<L>::NT);
+    switch (mode) {
+      case kRowFwd: hipLaunchKernelGGL((k_row<L, kRowFwd>), grid, blk, 0, s, a); break;
+      case kRowInv: hipLaunchKernelGGL((k_row<L, kRowInv>), grid, blk, 0, s, a); break;
+      case kRowAbsFwd: hipLaunchKernelGGL((k_row<L, kRowAbsFwd>), grid, blk, 0, s, a); break;
+      case kRowBlurOut: hipLaunchKernelGGL((k_row<L, kRowBlurOut>), grid, blk, 0, s, a); break;
+    }
+  }
+  static void update(const UpdateArgs& a, bool dm, hipStream_t s) {
+    const dim3 grid((a.nlines + RowShape<L>::M - 1) / RowShape<L>::M), blk(RowShape<L>::NT);
+    if (dm) hipLaunchKernelGGL((k_row_update<L, true>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_row_update<L, false>), grid, blk, 0, s, a);
+  }
+  static int col_blocks(const LineGeom& g) {
+    return ((g.ncontig + ColShape<L>::C - 1) / ColShape<L>::C) * g.nouter;
+  }
+  static void col(int mode, const ColArgs& a, hipStream_t s) {
+    const dim3 grid(col_blocks(a.geo)), blk(ColShape<L>::NT);
+    switch (mode) {
+      case kColFwd: hipLaunchKernelGGL((k_col<L, kColFwd>), grid, blk, 0, s, a); break;
+      case kColInv: hipLaunchKernelGGL((k_col<L, kColInv>), grid, blk, 0, s, a); break;
+      case kColProject: hipLaunchKernelGGL((k_col<L, kColProject>), grid, blk, 0, s, a); break;
+      case kColEFourier: hipLaunchKernelGGL((k_col<L, kColEFourier>), grid, blk, 0, s, a); break;
+      case kColBlur: hipLaunchKernelGGL((k_col<L, kColBlur>), grid, blk, 0, s, a); break;
+    }
+  }
+};
+
+// Dispatch table entry for one length.
+struct LengthEntry {
+  int L;
+  int twtotal;
+  void (*row)(int, const RowArgs&, hipStream_t);
+  void (*update)(const UpdateArgs&, bool, hipStream_t);
+  void (*col)(int, const ColArgs&, hipStream_t);
+  int (*col_blocks)(const LineGeom&);
+};
+
+template <int L>
+constexpr LengthEntry length_entry() {
+  return LengthEntry{L, PlanOf<L>::P.twtotal, &LengthOps<L>::row, &LengthOps<L>::update,
+                     &LengthOps<L>::col, &LengthOps<L>::col_blocks};
+}
+
+// Registry implemented by the instantiation units.
+const LengthEntry* find_length(int L);
+
+}  // namespace spx

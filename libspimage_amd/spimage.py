@@ -1,0 +1,295 @@
+"""ctypes mirror of the reference's ``spimage`` Python module for the phasing path.
+
+The reference builds ``spimage_pybackend`` with SWIG from its headers
+(reference src/spimage_pybackend.i:163-185) and re-exports it from
+``spimage.py`` (reference src/spimage.py:11).  This module binds the same C
+names from ``libspimage_hip.so`` so that code written against the reference
+reads the same: ``sp_phaser_alloc``, ``sp_phaser_init``,
+``sp_phaser_set_amplitudes``, ``sp_phaser_iterate``, ``sp_smap_alloc`` and so
+on.  Arrays cross the boundary as numpy views in the reference typemaps'
+orientation: ``(y, x)`` for 2-D images and ``(z, y, x)`` for 3-D ones
+(reference src/spimage_pybackend.i:79-101).
+
+The library is loaded from ``libspimage_amd/_lib``; importing this module
+raises if it has not been built, so nothing silently falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libspimage_hip.so"
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"{_LIB_PATH} is missing: build it with `python -m libspimage_amd.build` "
+        "(the HIP engine has no CPU fallback)"
+    )
+lib = C.CDLL(str(_LIB_PATH))
+
+# ---------------------------------------------------------------------------
+# enums (reference include/spimage/phasing.h:14-20, support_update.h:9)
+# ---------------------------------------------------------------------------
+SpModelRandomPhases, SpModelZeroPhases, SpModelRandomValues, SpModelMaskedOutZeroed = 1, 2, 4, 256
+SpSupportFromPatterson = 1
+SpHIO, SpRAAR, SpDiffMap, SpER = 1, 2, 3, 4
+SpNoConstraints, SpRealObject, SpPositiveRealObject, SpPositiveComplexObject = 0, 1, 2, 4
+SpPositivityFlipping, SpCentrosymmetricObject, SpAmplitudeErrorMargin = 8, 16, 32
+SpEngineAutomatic, SpEngineCPU, SpEngineCUDA = 0, 1, 2
+SpEngineHIP = 2
+SpPixelInsideSupport, SpPixelMeasuredAmplitude = 1, 2
+SpRecoverPhases, SpRecoverAmplitudes = 0, 1
+SpSupportThreshold, SpSupportArea, SpSupportCentreImage, SpSupportTemplate, SpSupportStatic, SpSupportClose = 1, 2, 3, 4, 5, 6
+SP_COPY_DATA, SP_COPY_DETECTOR, SP_COPY_MASK, SP_COPY_ALL = 1, 0, 2, 3
+
+
+# ---------------------------------------------------------------------------
+# structs (layouts of include/sp_core.h, sp_phasing.h, sp_support.h)
+# ---------------------------------------------------------------------------
+class Complex(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+class sp_3matrix(C.Structure):
+    _fields_ = [("x", C.c_uint), ("y", C.c_uint), ("z", C.c_uint), ("data", C.POINTER(C.c_float))]
+
+
+class sp_i3matrix(C.Structure):
+    _fields_ = [("x", C.c_uint), ("y", C.c_uint), ("z", C.c_uint), ("data", C.POINTER(C.c_int))]
+
+
+class sp_c3matrix(C.Structure):
+    _fields_ = [("x", C.c_uint), ("y", C.c_uint), ("z", C.c_uint), ("data", C.POINTER(Complex))]
+
+
+class Detector(C.Structure):
+    _fields_ = [
+        ("image_center", C.c_float * 3),
+        ("pixel_size", C.c_float * 3),
+        ("detector_distance", C.c_float),
+        ("wavelength", C.c_float),
+        ("orientation", C.c_void_p),
+    ]
+
+
+class Image(C.Structure):
+    _fields_ = [
+        ("phased", C.c_int),
+        ("scaled", C.c_int),
+        ("image", C.POINTER(sp_c3matrix)),
+        ("mask", C.POINTER(sp_i3matrix)),
+        ("detector", C.POINTER(Detector)),
+        ("shifted", C.c_int),
+        ("rec_coords", C.POINTER(sp_3matrix)),
+        ("num_dimensions", C.c_int),
+    ]
+
+
+class sp_list(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("used", C.c_uint), ("size", C.c_uint)]
+
+
+class sp_smap(C.Structure):
+    _fields_ = [("keys", C.POINTER(sp_list)), ("values", C.POINTER(sp_list))]
+
+
+class SpPhasingAlgorithm(C.Structure):
+    _fields_ = [("type", C.c_int), ("params", C.c_void_p)]
+
+
+class SpSupportAlgorithm(C.Structure):
+    _fields_ = [("type", C.c_int), ("params", C.c_void_p), ("function", C.c_void_p)]
+
+
+class SpSupportArray(C.Structure):
+    _fields_ = [("size", C.c_int), ("algorithms", C.POINTER(C.POINTER(SpSupportAlgorithm))), ("update_period", C.c_int)]
+
+
+class SpPhaser(C.Structure):
+    _fields_ = [
+        ("amplitudes", C.POINTER(sp_3matrix)),
+        ("amplitudes_min", C.POINTER(sp_3matrix)),
+        ("amplitudes_max", C.POINTER(sp_3matrix)),
+        ("phased_amplitudes", C.POINTER(sp_c3matrix)),
+        ("pixel_flags", C.POINTER(sp_i3matrix)),
+        ("phasing_objective", C.c_int),
+        ("algorithm", C.POINTER(SpPhasingAlgorithm)),
+        ("sup_algorithm", C.POINTER(SpSupportArray)),
+        ("iteration", C.c_int),
+        ("image_size", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("nz", C.c_int),
+        ("model", C.POINTER(Image)),
+        ("model_iteration", C.c_int),
+        ("old_model", C.POINTER(Image)),
+        ("old_model_iteration", C.c_int),
+        ("model_change", C.POINTER(Image)),
+        ("model_change_iteration", C.c_int),
+        ("support", C.POINTER(Image)),
+        ("support_iteration", C.c_int),
+        ("amplitudes_image", C.POINTER(Image)),
+        ("fmodel", C.POINTER(Image)),
+        ("fmodel_iteration", C.c_int),
+        ("model_before_projection", C.POINTER(Image)),
+        ("model_before_projection_iteration", C.c_int),
+        ("engine", C.c_int),
+        ("g0", C.POINTER(Image)),
+        ("g1", C.POINTER(Image)),
+        ("gp", C.POINTER(Image)),
+        ("hip_engine", C.c_void_p),
+    ]
+
+
+PImage = C.POINTER(Image)
+PSmap = C.POINTER(sp_smap)
+PPhaser = C.POINTER(SpPhaser)
+PAlg = C.POINTER(SpPhasingAlgorithm)
+PSupAlg = C.POINTER(SpSupportAlgorithm)
+PSupArr = C.POINTER(SpSupportArray)
+
+
+def _bind(name, restype, *argtypes, alias=None):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    globals()[alias or name] = f
+    return f
+
+
+# images
+_bind("_sp_image_alloc", PImage, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int)
+_bind("_sp_image_free", None, PImage, C.c_char_p, C.c_int)
+_bind("_sp_image_duplicate", PImage, PImage, C.c_int, C.c_char_p, C.c_int)
+
+
+def sp_image_alloc(x: int, y: int, z: int) -> PImage:
+    return _sp_image_alloc(x, y, z, b"python", 0)
+
+
+def sp_image_free(img: PImage) -> None:
+    _sp_image_free(img, b"python", 0)
+
+
+def sp_image_duplicate(img: PImage, flags: int) -> PImage:
+    return _sp_image_duplicate(img, flags, b"python", 0)
+
+
+_bind("p_drand48", C.c_float)
+_bind("sp_srand", None, C.c_int)
+
+# schedules: the header's inline sp_smap_* map onto these exports
+_bind("spx_smap_alloc", PSmap, C.c_uint, alias="sp_smap_alloc")
+_bind("spx_smap_free", None, PSmap, alias="sp_smap_free")
+_bind("spx_smap_insert", None, PSmap, C.c_float, C.c_float, alias="sp_smap_insert")
+_bind("spx_smap_create_from_pair", PSmap, C.c_float, C.c_float, alias="sp_smap_create_from_pair")
+_bind("spx_smap_interpolate", C.c_float, PSmap, C.c_float, alias="sp_smap_interpolate")
+_bind("spx_smap_max", C.c_float, PSmap, alias="sp_smap_max")
+_bind("spx_smap_min", C.c_float, PSmap, alias="sp_smap_min")
+
+# algorithms
+_bind("sp_phasing_hio_alloc", PAlg, PSmap, C.c_int)
+_bind("sp_phasing_raar_alloc", PAlg, PSmap, C.c_int)
+_bind("sp_phasing_diff_map_alloc", PAlg, PSmap, C.c_float, C.c_float, C.c_int)
+_bind("sp_phasing_er_alloc", PAlg, C.c_int)
+
+# phaser
+_bind("sp_phaser_alloc", PPhaser)
+_bind("sp_phaser_free", None, PPhaser)
+for _n in ("model", "model_with_support", "model_before_projection", "old_model", "fmodel",
+           "fmodel_with_mask", "model_change", "support", "amplitudes"):
+    _bind(f"sp_phaser_{_n}", PImage, PPhaser)
+_bind("sp_phaser_ereal", C.c_float, PPhaser)
+_bind("sp_phaser_efourier", C.c_float, PPhaser, C.POINTER(C.c_float))
+_bind("sp_phaser_support_fraction", C.c_float, PPhaser)
+for _n in ("model", "support", "phased_amplitudes", "amplitudes"):
+    _bind(f"sp_phaser_set_{_n}", None, PPhaser, PImage)
+_bind("sp_phaser_set_amplitudes_margins", None, PPhaser, PImage, PImage)
+_bind("sp_phaser_set_objective", None, PPhaser, C.c_int)
+_bind("sp_phaser_init", C.c_int, PPhaser, PAlg, PSupArr, C.c_int)
+_bind("sp_phaser_set_phasing_algorithm", C.c_int, PPhaser, PAlg)
+_bind("sp_phaser_set_support_algorithm", C.c_int, PPhaser, PSupArr)
+_bind("sp_phaser_init_model", C.c_int, PPhaser, PImage, C.c_int)
+_bind("sp_phaser_init_support", C.c_int, PPhaser, PImage, C.c_int, C.c_float)
+_bind("sp_phaser_iterate", C.c_int, PPhaser, C.c_int)
+
+# support
+_bind("sp_support_threshold_alloc", PSupAlg, PSmap, PSmap)
+_bind("sp_support_area_alloc", PSupAlg, PSmap, PSmap)
+_bind("sp_support_static_alloc", PSupAlg)
+_bind("sp_support_close_alloc", PSupAlg, C.c_int)
+_bind("sp_support_centre_image_alloc", PSupAlg)
+_bind("sp_support_array_alloc", PSupArr, C.c_int, C.c_int)
+_bind("sp_support_array_init", PSupArr, PSupAlg, C.c_int)
+_bind("sp_support_array_set", None, PSupArr, C.c_int, PSupAlg)
+_bind("sp_support_array_append", None, PSupArr, PSupAlg)
+_bind("sp_support_array_update", C.c_int, PSupArr, PPhaser)
+
+# FFT and blur
+_bind("sp_image_fft", PImage, PImage)
+_bind("sp_image_ifft", PImage, PImage)
+_bind("sp_image_fft_fast", None, PImage, PImage)
+_bind("sp_image_ifft_fast", None, PImage, PImage)
+_bind("sp_gaussian_blur", PImage, PImage, C.c_float)
+
+# plain-pointer engine ABI
+_bind("spx_device_available", C.c_int)
+_bind("spx_last_error", C.c_char_p)
+_bind("spx_supported_length", C.c_int, C.c_int)
+_bind("spx_fft_device", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
+_bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float))
+_bind("spx_phaser_device_arrays", C.c_int, PPhaser, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+      C.POINTER(C.c_void_p))
+_bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)
+
+
+# ---------------------------------------------------------------------------
+# numpy helpers (the reference typemaps' orientation)
+# ---------------------------------------------------------------------------
+def _shape(img: PImage) -> tuple[int, ...]:
+    m = img.contents.image.contents
+    return (m.y, m.x) if m.z == 1 else (m.z, m.y, m.x)
+
+
+def image_view(img: PImage) -> np.ndarray:
+    """complex64 view of an Image's data, (y, x) or (z, y, x)."""
+    m = img.contents.image.contents
+    n = m.x * m.y * m.z
+    buf = C.cast(m.data, C.POINTER(C.c_float * (2 * n))).contents
+    return np.frombuffer(buf, dtype=np.complex64).reshape(_shape(img))
+
+
+def mask_view(img: PImage) -> np.ndarray:
+    m = img.contents.mask.contents
+    n = m.x * m.y * m.z
+    buf = C.cast(m.data, C.POINTER(C.c_int * n)).contents
+    return np.frombuffer(buf, dtype=np.int32).reshape(_shape(img))
+
+
+def image_from_array(data: np.ndarray, mask: np.ndarray | None = None) -> PImage:
+    """New Image holding `data` ((y, x) or (z, y, x)) and `mask`."""
+    a = np.asarray(data)
+    if a.ndim == 2:
+        z, (y, x) = 1, a.shape
+    elif a.ndim == 3:
+        z, y, x = a.shape
+    else:
+        raise ValueError("images are 2-D or 3-D")
+    img = sp_image_alloc(x, y, z)
+    image_view(img)[...] = a.reshape(_shape(img)).astype(np.complex64)
+    if mask is not None:
+        mask_view(img)[...] = np.asarray(mask).reshape(_shape(img)).astype(np.int32)
+    return img
+
+
+def smap(pairs) -> PSmap:
+    """sp_smap from (key, value) pairs."""
+    m = sp_smap_alloc(max(1, len(pairs)))
+    for k, v in pairs:
+        sp_smap_insert(m, float(k), float(v))
+    return m
+
+
+def last_error() -> str:
+    return spx_last_error().decode(errors="replace")
