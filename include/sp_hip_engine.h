@@ -42,6 +42,11 @@ spimage_EXPORT int spx_phaser_time_iterations(SpPhaser *ph, int iterations, floa
  * and model before projection gp (complex64, image_size elements each). */
 spimage_EXPORT int spx_phaser_device_arrays(SpPhaser *ph, void **g1, void **g0, void **gp);
 
+/* Threshold applied by the phaser's last area / threshold / Patterson
+ * support update (the value the reference computes at
+ * src/support_update.c:144,175 and src/phasing.c:754). */
+spimage_EXPORT int spx_phaser_last_threshold(SpPhaser *ph, float *threshold);
+
 /* Seed of the engine's counter-based generator used by
  * sp_phaser_init_model(..., SpModelRandomPhases / SpModelRandomValues). */
 spimage_EXPORT void spx_phaser_set_seed(SpPhaser *ph, unsigned long long seed);

@@ -346,6 +346,9 @@ SpPhasingAlgorithm* sp_phasing_er_alloc(SpPhasingConstraints c) {
 SpPhaser* sp_phaser_alloc(void) {
   SpPhaser* r = (SpPhaser*)calloc(1, sizeof(SpPhaser));
   r->model_iteration = -1;
+  // the reference leaves old_model_iteration at 0, so its old_model getter
+  // returns NULL before the first iteration; -1 makes it return g0 instead
+  r->old_model_iteration = -1;
   r->fmodel_iteration = -1;
   r->model_before_projection_iteration = -1;
   r->model_change_iteration = -1;
@@ -659,13 +662,12 @@ static Image* fmodel_nc(SpPhaser* ph) {
     ph->fmodel_iteration = ph->iteration;
     Engine* e = engine_of(ph);
     if (!e) return nullptr;
-    // the transform lands in gp's sibling scratch through the efourier pass
-    std::vector<float2> dummy;
+    // the efourier pass writes the transform of g1 into a temporary buffer
     float2* dev = nullptr;
     if (hipMalloc(&dev, sizeof(float2) * (size_t)ph->image_size) != hipSuccess) return nullptr;
     int rc = e->fmodel(dev);
     if (!rc) rc = e->download(dev, (float2*)ph->fmodel->image->data);
-    hipFree(dev);
+    (void)hipFree(dev);
     if (rc) {
       report("sp_phaser_fmodel");
       return nullptr;
@@ -705,7 +707,7 @@ Image* sp_phaser_model_change(SpPhaser* ph) {
     if (hipMalloc(&dev, sizeof(float2) * (size_t)ph->image_size) != hipSuccess) return nullptr;
     int rc = e->model_change(dev);
     if (!rc) rc = e->download(dev, (float2*)ph->model_change->image->data);
-    hipFree(dev);
+    (void)hipFree(dev);
     if (rc) {
       report("sp_phaser_model_change");
       return nullptr;
@@ -1004,6 +1006,12 @@ int spx_phaser_device_arrays(SpPhaser* ph, void** g1, void** g0, void** gp) {
   if (g0) *g0 = e->g0();
   if (gp) *gp = e->gp_buf();
   return 0;
+}
+
+int spx_phaser_last_threshold(SpPhaser* ph, float* t) {
+  Engine* e = engine_of(ph);
+  if (!e || !t) return -8;
+  return e->last_threshold(t);
 }
 
 void spx_phaser_set_seed(SpPhaser* ph, unsigned long long seed) {

@@ -84,6 +84,8 @@ class Engine {
   int shrinkwrap_area(float sigma, float area);
   int shrinkwrap_threshold(float sigma, float rel);
   int patterson_support(const float* amp_raw, float value);
+  // threshold used by the last area/threshold/Patterson support update
+  int last_threshold(float* t);
 
   // ---- plain transforms ----
   int fft_nd(const float2* in, float2* out, int dir);

@@ -30,7 +30,7 @@ bool device_available() {
   if (state < 0) {
     int n = 0;
     state = (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
-    if (state) hipSetDevice(0);
+    if (state) state = (hipSetDevice(0) == hipSuccess) ? 1 : 0;
   }
   return state == 1;
 }
@@ -271,8 +271,10 @@ __global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, u
 
 // support bit = |blur| > threshold (reference src/support_update.c:392-400);
 // mode 0: threshold = selected value, mode 1: max * rel (src/support_update.c:175)
-__global__ void k_mask(const float* absb, uint8_t* flags, long n, const unsigned* ctl, int mode, float rel) {
+__global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl, int mode, float rel) {
   const float t = (mode == 0) ? __uint_as_float(ctl[kCtlThresh]) : __uint_as_float(ctl[kCtlMax]) * rel;
+  // record the threshold of mode 1 (mode 0 already holds it in kCtlThresh)
+  if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlThresh] = __float_as_uint(t);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const uint8_t f = flags[i];
     flags[i] = (absb[i] > t) ? (uint8_t)(f | kInsideSupport) : (uint8_t)(f & ~kInsideSupport);
@@ -363,7 +365,7 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
     src = out;
   }
   if (src == in && in != out)
-    hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz, hipMemcpyDeviceToDevice, s);
 }
 
 // Column work after the x pass: the last non-trivial axis gets `mode`; the
@@ -473,13 +475,14 @@ int Engine::alloc_all() {
 }
 
 Engine::~Engine() {
-  if (s) hipStreamSynchronize(s);
+  // teardown: errors here have nowhere to go
+  if (s) (void)hipStreamSynchronize(s);
   void* ptrs[] = {g[0], g[1], W, Wf, gp, S1, ampx, amp_min, amp_max, absb, phased, flags, partials, red, hist, ctl};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (ev0) hipEventDestroy(ev0);
-  if (ev1) hipEventDestroy(ev1);
-  if (s) hipStreamDestroy(s);
+    if (p) (void)hipFree(p);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (s) (void)hipStreamDestroy(s);
 }
 
 // encoded amplitudes: the raw values pass through `absb` (scratch) once
@@ -796,6 +799,14 @@ int Engine::shrinkwrap_area(float sigma, float area) {
   return 0;
 }
 
+int Engine::last_threshold(float* t) {
+  unsigned bits = 0;
+  SPX_HIP(hipMemcpyAsync(&bits, ctl + kCtlThresh, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  memcpy(t, &bits, sizeof(float));
+  return 0;
+}
+
 int Engine::shrinkwrap_threshold(float sigma, float rel) {
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, 0u);
   if (int rc = blur_to_abs(sigma, nullptr, ctl + kCtlMax)) return rc;
@@ -895,7 +906,7 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
   if (!rc && hipGetLastError() != hipSuccess) rc = -20;
   if (!rc && hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -20;
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = -20;
-  hipFree(d);
+  (void)hipFree(d);
   if (rc == -20) set_error("HIP error in one-shot transform");
   return rc;
 }

@@ -242,6 +242,7 @@ _bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_flo
 _bind("spx_phaser_device_arrays", C.c_int, PPhaser, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
       C.POINTER(C.c_void_p))
 _bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)
+_bind("spx_phaser_last_threshold", C.c_int, PPhaser, C.POINTER(C.c_float))
 
 
 # ---------------------------------------------------------------------------
