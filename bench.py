@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the HIP phasing engine on one MI355X.
+
+Headline: HIO iterations per second on a 1024 x 1024 complex64 pattern (the
+seeded blob pattern of libspimage_amd.workloads.blob_1024, static support,
+beta = 0.9), inputs resident in HBM, timed with HIP events on the engine's
+stream.  A step is one HIO iteration: the fused Fourier-space pass (column
+FFT, modulus projection, inverse column FFT) and the fused real-space pass
+(inverse row FFT, 1/N, HIO update, support constraint, forward row FFT of the
+next iterate).
+
+Prints one JSON line with the roofline of the dominant pass (algorithmic
+bytes from the 78 N bytes of a 2-D HIO iteration: 41 N for the real-space
+pass, 37 N for the Fourier-space pass), the whole-iteration roofline, the
+1024^2 RAAR + shrinkwrap reconstruction (300 iterations, area support every
+20) and a CPU baseline: the oracle's restatement of the reference loop on one
+host core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_HIO = 78             # algorithmic bytes per pixel of one 2-D HIO iteration
+BYTES_REAL_PASS = 41       # inverse + forward x passes (32) + g0 (8) + support mask (1)
+BYTES_FOURIER_PASS = 37    # forward + inverse y passes (32) + amplitudes (4) + measured mask (1)
+MKL_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.2", "/usr/lib/x86_64-linux-gnu/libmkl_rt.so"]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def make_hio_phaser(sp, pattern, beta=0.9, alg="hio", sup_array=None):
+    keep = []
+    bmap = sp.smap([(0, beta)])
+    keep.append(bmap)
+    a = sp.sp_phasing_hio_alloc(bmap, 0) if alg == "hio" else sp.sp_phasing_raar_alloc(bmap, 0)
+    ph = sp.sp_phaser_alloc()
+    rc = sp.sp_phaser_init(ph, a, sup_array, sp.SpEngineHIP)
+    if rc != 0:
+        raise RuntimeError(f"sp_phaser_init failed ({rc}): {sp.last_error()}")
+    imgs = [sp.image_from_array(pattern.amplitudes, pattern.mask),
+            sp.image_from_array(pattern.start),
+            sp.image_from_array(pattern.support)]
+    sp.sp_phaser_set_amplitudes(ph, imgs[0])
+    if sp.sp_phaser_init_model(ph, imgs[1], 0) != 0:
+        raise RuntimeError(sp.last_error())
+    if sp.sp_phaser_init_support(ph, imgs[2], 0, 0.0) != 0:
+        raise RuntimeError(sp.last_error())
+    return ph, (a, imgs, keep)
+
+
+def gpu_bench(args):
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import workloads
+    if not sp.spx_device_available():
+        raise SystemExit("bench.py: no HIP device visible")
+    p = workloads.blob_1024()
+    N = p.amplitudes.size
+    ph, keep = make_hio_phaser(sp, p)
+    ms = C.c_float()
+    if sp.spx_phaser_time_iterations(ph, args.warmup, C.byref(ms)) != 0:
+        raise RuntimeError(sp.last_error())
+    if sp.spx_phaser_time_iterations(ph, args.steps, C.byref(ms)) != 0:
+        raise RuntimeError(sp.last_error())
+    t_iter = ms.value / 1e3 / args.steps
+    # per-pass durations (events between the two passes of each iteration)
+    mf, mr = C.c_float(), C.c_float()
+    npass = max(200, args.steps // 4)
+    if sp.spx_phaser_time_passes(ph, npass, C.byref(mf), C.byref(mr)) != 0:
+        raise RuntimeError(sp.last_error())
+    t_f, t_r = mf.value / 1e3 / npass, mr.value / 1e3 / npass
+    sp.sp_phaser_free(ph)
+
+    passes = {
+        "real_space_pass": {"bytes_per_pixel": BYTES_REAL_PASS, "seconds": t_r},
+        "fourier_space_pass": {"bytes_per_pixel": BYTES_FOURIER_PASS, "seconds": t_f},
+    }
+    dom_name = max(passes, key=lambda k: passes[k]["seconds"])
+    dom = passes[dom_name]
+    achieved = dom["bytes_per_pixel"] * N / dom["seconds"]
+    traffic = None
+    tfile = ROOT / "profiles" / "hio_1024_hbm_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom_name)
+        except (ValueError, OSError):
+            traffic = None
+    roofline = {
+        "bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+        "frac": achieved / PEAK_HBM, "traffic": traffic,
+        "algorithmic_bytes": dom["bytes_per_pixel"] * N, "avg_launch_us": dom["seconds"] * 1e6,
+    }
+    it_achieved = BYTES_HIO * N / t_iter
+    iteration_roofline = {
+        "achieved": it_achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": it_achieved / PEAK_HBM,
+        "algorithmic_bytes": BYTES_HIO * N,
+        "passes_us": {k: v["seconds"] * 1e6 for k, v in passes.items()},
+    }
+
+    # the 1024^2 reconstruction: RAAR beta 0.87, 300 iterations, area
+    # shrinkwrap every 20 (blur 3.0 -> 0.7 px, 8 % of the pixels)
+    bmap, amap = sp.smap([(0, 3.0), (300, 0.7)]), sp.smap([(0, 0.08)])
+    salg = sp.sp_support_area_alloc(bmap, amap)
+    sarr = sp.sp_support_array_init(salg, 20)
+    ph2, keep2 = make_hio_phaser(sp, p, beta=0.87, alg="raar", sup_array=sarr)
+    sp.sp_phaser_iterate(ph2, 20)  # warm the shrinkwrap kernels
+    sp.sp_phaser_efourier(ph2, None)
+    ph3, keep3 = make_hio_phaser(sp, p, beta=0.87, alg="raar", sup_array=sarr)
+    sp.sp_phaser_efourier(ph3, None)  # synchronises the stream
+    t0 = time.perf_counter()
+    rc = sp.sp_phaser_iterate(ph3, 300)
+    fc = C.c_float()
+    ef = sp.sp_phaser_efourier(ph3, C.byref(fc))  # synchronising read-back
+    t_rec = time.perf_counter() - t0
+    recon = {"workload": "RAAR 1024x1024, beta 0.87, 300 iterations, area shrinkwrap every 20",
+             "rc": rc, "seconds": t_rec, "iter_per_s": 300 / t_rec, "final_efourier": ef,
+             "support_fraction": sp.sp_phaser_support_fraction(ph3)}
+    sp.sp_phaser_free(ph2)
+    sp.sp_phaser_free(ph3)
+    return t_iter, roofline, iteration_roofline, recon
+
+
+def cpu_baseline_worker(seconds: float):
+    """Runs in a child process: the oracle's HIO loop on one core."""
+    from libspimage_amd import workloads
+    from oracle.oracle import Oracle
+    o = Oracle("f32")
+    backend = "own double-precision FFT"
+    for path in MKL_CANDIDATES:
+        if os.path.exists(path) and o.set_fft_backend(1, path) == 0:
+            backend = f"FFTW3 API of {os.path.basename(path)} (MKL, sequential), plan per call"
+            break
+    p = workloads.blob_1024()
+    ph = o.phaser(p.amplitudes.shape)
+    ph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
+    alg = o.alg(1, 0, beta=[(0, 0.9)])
+    ph.iterate(alg, 1)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        ph.iterate(alg, 1)
+        n += 1
+    dt = time.perf_counter() - t0
+    print(json.dumps({"iters": n, "seconds": dt, "backend": backend}))
+
+
+def cpu_baseline(seconds: float):
+    env = dict(os.environ, MKL_NUM_THREADS="1", MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-baseline-only", "--cpu-seconds", str(seconds)],
+                       capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return {"value": None, "unit": "iter/s", "cores": 1, "kind": "port", "sample": f"failed: {r.stderr[-300:]}"}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": d["iters"] / d["seconds"], "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": f"{d['iters']} HIO iterations of the 1024x1024 blob pattern in {d['seconds']:.1f} s "
+                      f"(oracle restatement of src/phasing.c:1142-1172, FFT: {d['backend']})"}
+
+
+def main():
+    args = parse()
+    if args.cpu_baseline_only:
+        cpu_baseline_worker(args.cpu_seconds)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent reconstructions shard by id across GPUs with no exchange)", file=sys.stderr)
+        raise SystemExit(2)
+    t_iter, roofline, it_roof, recon = gpu_bench(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
+    out = {
+        "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
+        "value": 1.0 / t_iter,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_iter * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "HIO 1024x1024 complex64, seeded blob pattern (1e7 photons, beamstop 8 px), "
+                               "static support, beta 0.9", "shape": [1024, 1024], "algorithm": "HIO"},
+        "roofline": roofline,
+        "iteration_roofline": it_roof,
+        "reconstruction_1024": recon,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,12 @@ spimage_EXPORT int spx_fft_device(void *d_in, void *d_out, int nx, int ny, int n
  * measured with HIP events on the engine's stream. */
 spimage_EXPORT int spx_phaser_time_iterations(SpPhaser *ph, int iterations, float *ms);
 
+/* The same iterations with HIP events around each pass: total milliseconds
+ * spent in the Fourier-space passes (column transforms with the modulus
+ * projection) and in the real-space passes (row transforms with the
+ * ER/HIO/RAAR/DM update). */
+spimage_EXPORT int spx_phaser_time_passes(SpPhaser *ph, int iterations, float *ms_fourier, float *ms_real);
+
 /* Device addresses of the phaser's current iterate g1, previous iterate g0
  * and model before projection gp (complex64, image_size elements each). */
 spimage_EXPORT int spx_phaser_device_arrays(SpPhaser *ph, void **g1, void **g0, void **gp);

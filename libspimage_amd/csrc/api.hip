@@ -976,25 +976,45 @@ int spx_fft_device(void* d_in, void* d_out, int nx, int ny, int nz, int directio
   return spx::fft_device((const float2*)d_in, (float2*)d_out, nx, ny, nz, direction);
 }
 
-int spx_phaser_time_iterations(SpPhaser* ph, int iterations, float* ms) {
-  Engine* e = engine_of(ph);
-  if (!e || !ph->algorithm) {
-    spx::set_error("phaser has no HIP engine or algorithm");
-    return -8;
-  }
+static spx::IterParams iter_params(SpPhaser* ph) {
   spx::IterParams p{};
   p.alg = ph->algorithm->type;
   p.objective = ph->phasing_objective;
-  if (p.alg == SpER) p.constraints = ((SpPhasingERParameters*)ph->algorithm->params)->constraints;
-  else if (p.alg == SpDiffMap) {
+  if (p.alg == SpER) {
+    p.constraints = ((SpPhasingERParameters*)ph->algorithm->params)->constraints;
+  } else if (p.alg == SpDiffMap) {
     SpPhasingDiffMapParameters* d = (SpPhasingDiffMapParameters*)ph->algorithm->params;
     p.constraints = d->constraints;
     p.gamma1 = d->gamma1;
     p.gamma2 = d->gamma2;
-  } else p.constraints = ((SpPhasingHIOParameters*)ph->algorithm->params)->constraints;
+  } else {
+    p.constraints = ((SpPhasingHIOParameters*)ph->algorithm->params)->constraints;
+  }
+  return p;
+}
+
+int spx_phaser_time_iterations(SpPhaser* ph, int iterations, float* ms) {
+  Engine* e = engine_of(ph);
+  if (!e || !ph->algorithm || !ms) {
+    spx::set_error("phaser has no HIP engine or algorithm");
+    return -8;
+  }
   std::vector<float> betas((size_t)(iterations > 0 ? iterations : 1));
   for (int i = 0; i < iterations; ++i) betas[i] = beta_at(ph->algorithm, ph->iteration + i);
-  const int rc = e->time_iterations(p, betas.data(), iterations, ms);
+  const int rc = e->time_iterations(iter_params(ph), betas.data(), iterations, ms);
+  if (!rc) ph->iteration += iterations;
+  return rc;
+}
+
+int spx_phaser_time_passes(SpPhaser* ph, int iterations, float* ms_fourier, float* ms_real) {
+  Engine* e = engine_of(ph);
+  if (!e || !ph->algorithm || !ms_fourier || !ms_real) {
+    spx::set_error("phaser has no HIP engine or algorithm");
+    return -8;
+  }
+  std::vector<float> betas((size_t)(iterations > 0 ? iterations : 1));
+  for (int i = 0; i < iterations; ++i) betas[i] = beta_at(ph->algorithm, ph->iteration + i);
+  const int rc = e->time_passes(iter_params(ph), betas.data(), iterations, ms_fourier, ms_real);
   if (!rc) ph->iteration += iterations;
   return rc;
 }

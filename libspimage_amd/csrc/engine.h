@@ -1,14 +1,21 @@
 // Device state and operations of the HIP phasing engine (host side).
 //
-// One Engine holds everything a phaser keeps resident in HBM for one array
-// shape: the two iterates, the work buffers the fused passes stream through,
-// the encoded amplitudes, the pixel flags and the scratch of the reductions
-// and of the shrinkwrap select.  All work is enqueued on the engine's own
-// stream; only the explicit download/metric calls synchronise.
+// One Engine holds everything that `nrec` same-shape reconstructions keep
+// resident in HBM: the two iterates, the work buffers the fused passes stream
+// through, the encoded amplitudes, the pixel flags and the scratch of the
+// reductions and of the shrinkwrap select.  Reconstruction r occupies
+// elements [r N, (r+1) N) of every array, so a row pass sees nrec*ny*nz lines
+// and a column pass nrec outer blocks: a batch costs the same launches as one
+// reconstruction.  A phaser (sp_phaser_*) is an Engine with nrec = 1.
+//
+// All work is enqueued on the engine's own stream; only the explicit
+// download / metric calls synchronise.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
 #include <vector>
+
 #include "passes.h"
 
 namespace spx {
@@ -17,13 +24,13 @@ namespace spx {
 void set_error(const char* fmt, ...);
 const char* last_error();
 
-#define SPX_HIP(call)                                                      \
-  do {                                                                     \
-    hipError_t e_ = (call);                                                \
-    if (e_ != hipSuccess) {                                                \
-      ::spx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
-      return -20;                                                          \
-    }                                                                      \
+#define SPX_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      ::spx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);     \
+      return -20;                                                                                      \
+    }                                                                                                  \
   } while (0)
 
 bool device_available();
@@ -37,31 +44,41 @@ struct IterParams {
   float gamma1, gamma2;
 };
 
-// Radix-select control block kept on the device
-enum CtlSlot { kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlMax = 3, kCtlWords = 8 };
+// Lengths and line geometry of one (nz, ny, nx) shape stacked nrec times.
+struct Shape {
+  int nx, ny, nz, nrec;
+  const LengthEntry *ex, *ey, *ez;
+  const float2 *twx, *twy, *twz;
+  LineGeom gy, gz;
+};
+int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
 
 class Engine {
  public:
-  // Returns null (with spx_last_error set) if the shape is unsupported or
+  // Returns null (with the last error set) if the shape is unsupported or
   // device memory is short.
-  static Engine* create(int nx, int ny, int nz);
+  static Engine* create(int nx, int ny, int nz, int nrec = 1);
   ~Engine();
 
-  int nx = 0, ny = 0, nz = 0, ndim = 0;
-  long N = 0;
+  int nx = 0, ny = 0, nz = 0, nrec = 1;
+  long N = 0;   // pixels of one reconstruction
+  long NT = 0;  // pixels of all reconstructions
   hipStream_t s = nullptr;
 
-  // ---- host <-> device ----
-  int upload_amplitudes(const float* amp, const uint8_t* flags);  // also uploads flags
-  int upload_flags(const uint8_t* flags);
-  int download_flags(uint8_t* flags);
+  // ---- host <-> device (rec < 0: all reconstructions, host arrays of NT) ----
+  int upload_amplitudes(const float* amp, const uint8_t* flags, int rec = -1);  // also uploads flags
+  int upload_flags(const uint8_t* flags, int rec = -1);
+  int download_flags(uint8_t* flags, int rec = -1);
   int upload_margins(const float* amp_min, const float* amp_max);
   int upload_phased(const float2* phased, const uint8_t* flags);
-  int upload_model(const float2* model);       // into g1
-  int download(const float2* dev, float2* host);
+  int upload_model(const float2* model, int rec = -1);  // into g1
+  int download(const float2* dev, float2* host, int rec = -1);
   float2* g1() { return g[cur]; }
   float2* g0() { return g[cur ^ 1]; }
   float2* gp_buf() { return gp; }
+  float2* scratch() { return S1; }
+  // per-reconstruction algorithm / beta tables (null: uniform)
+  int set_rec_params(const int* alg, const float* beta);
 
   // ---- model initialisation (reference src/phasing.c:606-724) ----
   int init_model_from_amplitudes(const float* amp_raw, int flags, uint64_t seed);
@@ -69,30 +86,33 @@ class Engine {
   int finish_model_init(int flags);  // masked-out zeroing, g0 = gp = 0
 
   // ---- iterations ----
-  // Runs len(betas) iterations; writes gp on the last one when want_gp.
+  // Runs n iterations with betas[i] (ignored when per-reconstruction betas
+  // are set); writes gp on the last one when want_gp.
   int iterate(const IterParams& p, const float* betas, int n, bool want_gp);
   void invalidate_R() { R_valid = false; Rf_valid = false; }
 
-  // ---- metrics (reference src/phasing.c:803-893) ----
-  int efourier(double out[4], float2* fmodel_out /* device, may be null */);
+  // ---- metrics (reference src/phasing.c:803-893), nrec results each ----
+  int efourier(double* out /* 4 per rec */, float2* fmodel_out /* device, may be null */);
   int fmodel(float2* fmodel_out);
-  int ereal(double out[2]);
-  int support_count(double* out);
+  int ereal(double* out /* 2 per rec */);
+  int support_count(double* out /* 1 per rec */);
   int model_change(float2* out_dev);
 
   // ---- support updates (reference src/support_update.c:136-179) ----
   int shrinkwrap_area(float sigma, float area);
   int shrinkwrap_threshold(float sigma, float rel);
   int patterson_support(const float* amp_raw, float value);
-  // threshold used by the last area/threshold/Patterson support update
+  // threshold used by the last area/threshold/Patterson update (per rec)
   int last_threshold(float* t);
 
   // ---- plain transforms ----
   int fft_nd(const float2* in, float2* out, int dir);
-  int blur(const float2* in, float2* out, float sigma);  // sp_gaussian_blur on device data
 
   // timing of `n` iterations with HIP events on the engine stream
   int time_iterations(const IterParams& p, const float* betas, int n, float* ms);
+  // the same iterations with events around each pass: total milliseconds of
+  // the Fourier-space (column) passes and of the real-space (row) passes
+  int time_passes(const IterParams& p, const float* betas, int n, float* ms_fourier, float* ms_real);
 
   unsigned flags_version = 0;  // bumped whenever the support bits change
   uint64_t seed = 0x5EED5EEDull;
@@ -100,16 +120,13 @@ class Engine {
  private:
   Engine() = default;
   int alloc_all();
+  int iterate_impl(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks);
   int ensure_R();
   int ensure_Rf(float gamma1);
-  int fourier_pass(int mode_last, const float2* in, float2* out, ColArgs base);
-  int reduce_partials(int nparts, int nv, double* host_out);
-  int blur_to_abs(float sigma, unsigned* hist, unsigned* maxbits);
+  int reduce_per_rec(int parts_per_rec, int nv, double* host_out);
+  int blur_to_abs(float sigma, bool hist);
 
-  const LengthEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
-  const float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
-  LineGeom gy{}, gz{};
-
+  Shape sh{};
   float2* g[2] = {nullptr, nullptr};
   int cur = 0;
   float2 *W = nullptr, *Wf = nullptr, *gp = nullptr, *S1 = nullptr;
@@ -117,18 +134,18 @@ class Engine {
   float2* phased = nullptr;
   uint8_t* flags = nullptr;
   double* partials = nullptr;
-  int max_partials = 0;
   double* red = nullptr;
-  unsigned* hist = nullptr;   // 3 x 2048 bins
-  unsigned* ctl = nullptr;    // kCtlWords
-  bool have_margins = false, have_phased = false;
+  unsigned* hist = nullptr;   // kHistStride per rec
+  unsigned* ctl = nullptr;    // kCtlStride per rec
+  int* alg_rec = nullptr;     // optional per-rec tables
+  float* beta_rec = nullptr;
   bool R_valid = false, Rf_valid = false;
   unsigned Rf_version = 0;
   float Rf_gamma1 = 0.f;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-// process-wide helper stream/engine cache for the Image-level FFT calls
+// one-shot transforms for the Image-level API (host buffers)
 int fft_host(const float2* in, float2* out, int nx, int ny, int nz, int dir);
 int blur_host(const float2* in, float2* out, int nx, int ny, int nz, float sigma);
 // transform of device data on the library's stream (synchronous)

@@ -36,8 +36,9 @@ bool device_available() {
 }
 
 // ---------------------------------------------------------------------------
-// twiddle tables: per stage s of a length-L plan, entry j*(R-1)+(k-1) holds
-// exp(-2 pi i j k / (Ns R)), evaluated in double and rounded to float
+// twiddle tables: for stage s >= 1 of a length-L plan, entry
+// twoff[s] + j*tw_stride(R) + (k-1) holds exp(-2 pi i j k / (Ns R)),
+// evaluated in double and rounded to float (rows padded to an even length)
 // ---------------------------------------------------------------------------
 const float2* twiddles(int L) {
   static std::mutex mu;
@@ -46,13 +47,13 @@ const float2* twiddles(int L) {
   auto it = tables.find(L);
   if (it != tables.end()) return it->second;
   const FftPlan p = make_plan(L);
-  std::vector<float2> h(p.twtotal > 0 ? p.twtotal : 1);
-  for (int s = 0; s < p.n; ++s) {
+  std::vector<float2> h(p.twtotal > 0 ? p.twtotal : 2, make_float2(0.f, 0.f));
+  for (int s = 1; s < p.n; ++s) {
     const int R = p.r[s], Ns = p.ns[s];
     for (int j = 0; j < Ns; ++j)
       for (int k = 1; k < R; ++k) {
         const double a = -2.0 * M_PI * (double)j * (double)k / (double)(Ns * R);
-        h[p.twoff[s] + j * (R - 1) + (k - 1)] = make_float2((float)cos(a), (float)sin(a));
+        h[p.twoff[s] + j * tw_stride(R) + (k - 1)] = make_float2((float)cos(a), (float)sin(a));
       }
   }
   float2* d = nullptr;
@@ -67,20 +68,22 @@ const float2* twiddles(int L) {
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kRedBlocks = 512;
 constexpr int kRedThreads = 256;
 
+#define GRID_STRIDE(i, n) \
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
 __global__ void k_fill_c64(float2* p, long n, float2 v) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+  GRID_STRIDE(i, n) p[i] = v;
 }
 __global__ void k_fill_u32(unsigned* p, long n, unsigned v) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+  GRID_STRIDE(i, n) p[i] = v;
 }
 
 // amplitudes as the projection reads them: unmeasured pixels get the NaN
 // pattern kUnmeasuredBits, measured NaNs the canonical quiet NaN
 __global__ void k_encode_amp(const float* amp, const uint8_t* flags, float* ampx, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  GRID_STRIDE(i, n) {
     float a = amp[i];
     if (!(flags[i] & kMeasured)) a = __uint_as_float(kUnmeasuredBits);
     else if (isnan(a)) a = __uint_as_float(0x7FC00000u);
@@ -89,16 +92,23 @@ __global__ void k_encode_amp(const float* amp, const uint8_t* flags, float* ampx
 }
 
 __global__ void k_scale_copy(const float2* in, float2* out, long n, float s) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  GRID_STRIDE(i, n) {
     const float2 v = in[i];
     // sp_cmul(v, (s, 0)) as in sp_image_scale (reference src/image_util.c:1967-1970)
     out[i] = make_float2(v.x * s - v.y * 0.f, v.x * 0.f + s * v.y);
   }
 }
 
+// per-reconstruction scale factors (random-values initialisation)
+__global__ void k_scale_rec(float2* buf, long n_per_rec, long n, const float* s) {
+  GRID_STRIDE(i, n) {
+    const float f = s[i / n_per_rec];
+    buf[i] = make_float2(buf[i].x * f, buf[i].y * f);
+  }
+}
+
 __global__ void k_sub(const float2* a, const float2* b, float2* out, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
+  GRID_STRIDE(i, n) out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
 }
 
 // counter-based generator: splitmix64 finaliser of (seed, index, stream)
@@ -113,7 +123,7 @@ __device__ __forceinline__ double rng_u01(uint64_t seed, uint64_t i, uint64_t st
 // |amp| with a random (or zero) phase: sp_image_rephase (reference
 // src/image_util.c:1044-1067)
 __global__ void k_init_phases(const float* amp, float2* out, long n, uint64_t seed, int random) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  GRID_STRIDE(i, n) {
     const float a = fabsf(amp[i]);
     if (random) {
       const float phase = (float)(rng_u01(seed, i, 0) * M_PI * 2.0);
@@ -125,42 +135,48 @@ __global__ void k_init_phases(const float* amp, float2* out, long n, uint64_t se
 }
 
 __global__ void k_random_values(float2* out, long n, uint64_t seed) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = make_float2((float)((float)rng_u01(seed, i, 0) - 0.5), (float)((float)rng_u01(seed, i, 1) - 0.5));
+  GRID_STRIDE(i, n) out[i] = make_float2((float)((float)rng_u01(seed, i, 0) - 0.5),
+                                         (float)((float)rng_u01(seed, i, 1) - 0.5));
 }
 
 __global__ void k_amp2(const float* amp, float2* out, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = make_float2(amp[i] * amp[i], 0.f);
+  GRID_STRIDE(i, n) out[i] = make_float2(amp[i] * amp[i], 0.f);
 }
 
 __global__ void k_zero_unmeasured(float2* buf, const uint8_t* flags, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    if (!(flags[i] & kMeasured)) buf[i] = make_float2(0.f, 0.f);
+  GRID_STRIDE(i, n) if (!(flags[i] & kMeasured)) buf[i] = make_float2(0.f, 0.f);
 }
 
 // f1 of the difference map (reference src/phasing.c:1260-1274)
 __global__ void k_dm_f1(const float2* g, const uint8_t* flags, float gamma1, float2* out, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  GRID_STRIDE(i, n) {
     const float2 v = g[i];
     out[i] = (flags[i] & kInsideSupport) ? v : make_float2(-gamma1 * v.x, -gamma1 * v.y);
   }
 }
 
-// --- reductions: fixed grid, per-block partials, one-block final sum, so
-// --- the result does not depend on scheduling
+// --- per-reconstruction reductions: grid (blocks_per_rec, nrec); each block
+// --- strides over its reconstruction and writes NV partial sums; a second
+// --- kernel adds the partials of each reconstruction in a fixed order, so
+// --- results do not depend on scheduling
+#define REC_STRIDE(i, n_per_rec)                                                       \
+  const long rec_base_ = (long)blockIdx.y * (n_per_rec);                              \
+  for (long i = rec_base_ + blockIdx.x * (long)blockDim.x + threadIdx.x; i < rec_base_ + (n_per_rec); \
+       i += (long)gridDim.x * blockDim.x)
+
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&acc)[NV], double* partials) {
   __shared__ double red[NV * 32];
   block_sum<NV, kRedThreads>(acc, red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < NV; ++q) partials[blockIdx.x * NV + q] = acc[q];
+    for (int q = 0; q < NV; ++q) partials[((long)blockIdx.y * gridDim.x + blockIdx.x) * NV + q] = acc[q];
 }
 
 // E_R sums over the model before projection (reference src/phasing.c:826-832)
-__global__ __launch_bounds__(kRedThreads) void k_ereal(const float2* gp, const uint8_t* flags, long n, double* partials) {
+__global__ __launch_bounds__(kRedThreads) void k_ereal(const float2* gp, const uint8_t* flags, long n,
+                                                      double* partials) {
   double acc[2] = {0.0, 0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  REC_STRIDE(i, n) {
     const float2 v = gp[i];
     const double m = (double)sqrtf(v.y * v.y + v.x * v.x);
     if (!(flags[i] & kInsideSupport)) acc[0] += m * m;
@@ -171,7 +187,7 @@ __global__ __launch_bounds__(kRedThreads) void k_ereal(const float2* gp, const u
 
 __global__ __launch_bounds__(kRedThreads) void k_sumsq_c64(const float2* in, long n, double* partials) {
   double acc[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  REC_STRIDE(i, n) {
     const float2 v = in[i];
     acc[0] += (double)(v.y * v.y + v.x * v.x);
   }
@@ -180,49 +196,54 @@ __global__ __launch_bounds__(kRedThreads) void k_sumsq_c64(const float2* in, lon
 
 __global__ __launch_bounds__(kRedThreads) void k_sumsq_f32(const float* in, long n, double* partials) {
   double acc[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    acc[0] += (double)(in[i] * in[i]);
+  REC_STRIDE(i, n) acc[0] += (double)(in[i] * in[i]);
   write_partials<1>(acc, partials);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_count_support(const uint8_t* flags, long n, double* partials) {
   double acc[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    acc[0] += (flags[i] & kInsideSupport) ? 1.0 : 0.0;
+  REC_STRIDE(i, n) acc[0] += (flags[i] & kInsideSupport) ? 1.0 : 0.0;
   write_partials<1>(acc, partials);
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_blocks(const double* partials, int nblocks, int nv, double* out) {
-  __shared__ double sh[1024];
+// one block per reconstruction: out[rec*nv + q] = sum of its partials
+__global__ __launch_bounds__(256) void k_reduce_rec(const double* partials, int parts, int nv, double* out) {
+  __shared__ double sh[256];
+  const double* p = partials + (long)blockIdx.x * parts * nv;
   for (int q = 0; q < nv; ++q) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(long)b * nv + q];
+    for (int b = threadIdx.x; b < parts; b += blockDim.x) s += p[(long)b * nv + q];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[q] = sh[0];
+    if (threadIdx.x == 0) out[(long)blockIdx.x * nv + q] = sh[0];
     __syncthreads();
   }
 }
 
-// --- exact k-th largest |blur| (descending index k, the reference's
-// --- qsort-then-index, src/support_update.c:143-144) by a three-level radix
-// --- select over the float bit patterns (non-negative floats order like
-// --- their bits): 11 + 11 + 10 bits.
-__global__ void k_ctl_init(unsigned* ctl, unsigned k) {
-  if (threadIdx.x == 0) {
-    ctl[kCtlK] = k;
-    ctl[kCtlPrefix] = 0;
-    ctl[kCtlThresh] = 0;
-    ctl[kCtlMax] = 0;
+// --- exact k-th largest |blur| per reconstruction (descending index k, the
+// --- reference's qsort-then-index, src/support_update.c:143-144) by a
+// --- three-level radix select over the float bit patterns (non-negative
+// --- floats order like their bits): 11 + 11 + 10 bits.
+__global__ void k_ctl_init(unsigned* ctl, int nrec, unsigned k) {
+  for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+    unsigned* c = ctl + r * kCtlStride;
+    c[kCtlK] = k;
+    c[kCtlPrefix] = 0;
+    c[kCtlThresh] = 0;
+    c[kCtlMax] = 0;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_select(const unsigned* hist, int nbins, unsigned* ctl, int level) {
+// one block per reconstruction
+__global__ __launch_bounds__(1024) void k_select(const unsigned* hist_all, int level, unsigned* ctl_all) {
   __shared__ unsigned scan[1024];
+  const int nbins = (level == 3) ? 1024 : 2048;
+  const unsigned* hist = hist_all + (long)blockIdx.x * kHistStride + (level - 1) * 2048;
+  unsigned* ctl = ctl_all + blockIdx.x * kCtlStride;
   const int per = nbins / 1024;  // 1 or 2 bins per thread, walked high to low
   const int tid = threadIdx.x;
   const unsigned k = ctl[kCtlK];
@@ -237,26 +258,31 @@ __global__ __launch_bounds__(1024) void k_select(const unsigned* hist, int nbins
     __syncthreads();
   }
   unsigned before = scan[tid] - local;  // elements in strictly higher bins
+  const unsigned prefix = ctl[kCtlPrefix];
+  __syncthreads();
   for (int q = 0; q < per; ++q) {
     const int b = nbins - 1 - (tid * per + q);
     const unsigned c = hist[b];
     if (c > 0 && before <= k && k < before + c) {
       ctl[kCtlK] = k - before;
       if (level == 1) ctl[kCtlPrefix] = (unsigned)b;
-      else if (level == 2) ctl[kCtlPrefix] = (ctl[kCtlPrefix] << 11) | (unsigned)b;
-      else ctl[kCtlThresh] = (ctl[kCtlPrefix] << 10) | (unsigned)b;
+      else if (level == 2) ctl[kCtlPrefix] = (prefix << 11) | (unsigned)b;
+      else ctl[kCtlThresh] = (prefix << 10) | (unsigned)b;
     }
     before += c;
   }
 }
 
-__global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, unsigned* hist, const unsigned* ctl, int level) {
+// grid (blocks_per_rec, nrec)
+__global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, unsigned* hist_all, const unsigned* ctl_all,
+                                                    int level) {
   __shared__ unsigned h[2048];
   const int nb = (level == 2) ? 2048 : 1024;
+  unsigned* hist = hist_all + (long)blockIdx.y * kHistStride + (level - 1) * 2048;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0u;
   __syncthreads();
-  const unsigned pre = ctl[kCtlPrefix];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  const unsigned pre = ctl_all[blockIdx.y * kCtlStride + kCtlPrefix];
+  REC_STRIDE(i, n) {
     const unsigned b = __float_as_uint(absb[i]);
     if (level == 2) {
       if ((b >> 21) == pre) atomicAdd(&h[(b >> 10) & 2047u], 1u);
@@ -270,12 +296,14 @@ __global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, u
 }
 
 // support bit = |blur| > threshold (reference src/support_update.c:392-400);
-// mode 0: threshold = selected value, mode 1: max * rel (src/support_update.c:175)
-__global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl, int mode, float rel) {
+// mode 0: threshold = selected value, mode 1: max * rel
+// (src/support_update.c:175).  Grid (blocks_per_rec, nrec).
+__global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl_all, int mode, float rel) {
+  unsigned* ctl = ctl_all + blockIdx.y * kCtlStride;
   const float t = (mode == 0) ? __uint_as_float(ctl[kCtlThresh]) : __uint_as_float(ctl[kCtlMax]) * rel;
   // record the threshold of mode 1 (mode 0 already holds it in kCtlThresh)
   if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlThresh] = __float_as_uint(t);
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  REC_STRIDE(i, n) {
     const uint8_t f = flags[i];
     flags[i] = (absb[i] > t) ? (uint8_t)(f | kInsideSupport) : (uint8_t)(f & ~kInsideSupport);
   }
@@ -283,33 +311,36 @@ __global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl,
 
 inline dim3 ew_grid(long n) {
   long b = (n + 255) / 256;
-  if (b > 4096) b = 4096;
+  if (b > 8192) b = 8192;
   if (b < 1) b = 1;
   return dim3((unsigned)b);
+}
+
+// blocks per reconstruction of the per-reconstruction kernels (~4096
+// elements each, 8..512)
+inline int rec_blocks(long n) {
+  long b = (n + 4095) / 4096;
+  if (b < 8) b = 8;
+  if (b > 512) b = 512;
+  return (int)b;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// shape helpers shared by the engine and the one-shot transforms
+// shapes and pass sequences shared by the engine and the one-shot transforms
 // ---------------------------------------------------------------------------
-struct Shape {
-  int nx, ny, nz;
-  const LengthEntry *ex, *ey, *ez;
-  const float2 *twx, *twy, *twz;
-  LineGeom gy, gz;
-};
-
-static int make_shape(int nx, int ny, int nz, Shape* sh) {
+int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   memset(sh, 0, sizeof(*sh));
   sh->nx = nx;
   sh->ny = ny;
   sh->nz = nz;
-  if (nx < 1 || ny < 1 || nz < 1) {
-    set_error("invalid shape %d x %d x %d", nx, ny, nz);
+  sh->nrec = nrec;
+  if (nx < 1 || ny < 1 || nz < 1 || nrec < 1) {
+    set_error("invalid shape %d x %d x %d (x %d)", nx, ny, nz, nrec);
     return -11;
   }
-  if ((long)nx * ny * nz >= (1L << 31)) {
+  if ((long)nx * ny * nz >= (1L << 31) || (long)nx * ny * nz * nrec >= (1L << 40)) {
     set_error("array of %ld elements exceeds the int range of the phasing API", (long)nx * ny * nz);
     return -11;
   }
@@ -329,22 +360,29 @@ static int make_shape(int nx, int ny, int nz, Shape* sh) {
       return -20;
     }
   }
-  sh->gy = LineGeom{nx, nz, (long)nx * ny, (long)nx};
-  sh->gz = LineGeom{nx * ny, 1, 0L, (long)nx * ny};
+  // y lines: (rec, z) outer blocks of nx adjacent lines; z lines: rec outer
+  // blocks of nx*ny adjacent lines
+  sh->gy = LineGeom{nx, nz * nrec, (long)nx * ny, (long)nx};
+  sh->gz = LineGeom{nx * ny, nrec, (long)nx * ny * nz, (long)nx * ny};
   return 0;
+}
+
+static RowArgs row_args(const Shape& sh, const float2* in, float2* out) {
+  RowArgs r{};
+  r.in = in;
+  r.out = out;
+  r.tw = sh.twx;
+  r.nlines = sh.ny * sh.nz * sh.nrec;
+  r.lines_per_rec = sh.ny * sh.nz;
+  r.L = sh.nx;
+  return r;
 }
 
 // FFT over all non-trivial axes: x by a row pass, then y and z by column passes
 static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
   const float2* src = in;
   if (sh.ex) {
-    RowArgs r{};
-    r.in = in;
-    r.out = out;
-    r.tw = sh.twx;
-    r.nlines = sh.ny * sh.nz;
-    r.L = sh.nx;
-    sh.ex->row(dir < 0 ? kRowFwd : kRowInv, r, s);
+    sh.ex->row(dir < 0 ? kRowFwd : kRowInv, row_args(sh, in, out), s);
     src = out;
   }
   ColArgs c{};
@@ -365,7 +403,8 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
     src = out;
   }
   if (src == in && in != out)
-    (void)hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz * sh.nrec, hipMemcpyDeviceToDevice,
+                         s);
 }
 
 // Column work after the x pass: the last non-trivial axis gets `mode`; the
@@ -373,7 +412,6 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
 // after it.  For kColEFourier nothing is written back.
 static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2* out, ColArgs base, hipStream_t s) {
   const bool three = sh.ez && sh.ey;
-  const LengthEntry* last = sh.ez ? sh.ez : sh.ey;
   ColArgs a = base;
   if (three) {
     a.in = in;
@@ -400,43 +438,40 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     a.W = out;
     a.tw = sh.ez ? sh.twz : sh.twy;
     a.geo = sh.ez ? sh.gz : sh.gy;
-    last->col(mode, a, s);
+    (sh.ez ? sh.ez : sh.ey)->col(mode, a, s);
   }
 }
 
-static int last_axis_blocks(const Shape& sh) {
+// column tiles per reconstruction of the last axis (the efourier partials)
+static int last_axis_tiles_per_rec(const Shape& sh) {
   const LengthEntry* last = sh.ez ? sh.ez : sh.ey;
-  return last->col_blocks(sh.ez ? sh.gz : sh.gy);
+  LineGeom g = sh.ez ? sh.gz : sh.gy;
+  g.nouter = sh.ez ? 1 : sh.nz;  // one reconstruction
+  return last->col_blocks(g);
 }
 
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
-Engine* Engine::create(int nx, int ny, int nz) {
+Engine* Engine::create(int nx, int ny, int nz, int nrec) {
   if (!device_available()) {
     set_error("no HIP device available");
     return nullptr;
   }
   Shape sh;
-  if (make_shape(nx, ny, nz, &sh)) return nullptr;
+  if (make_shape(nx, ny, nz, nrec, &sh)) return nullptr;
   if (!sh.ex || (!sh.ey && !sh.ez)) {
     set_error("the HIP engine phases 2-D and 3-D arrays with nx > 1 (got %d x %d x %d)", nx, ny, nz);
     return nullptr;
   }
   Engine* e = new Engine();
+  e->sh = sh;
   e->nx = nx;
   e->ny = ny;
   e->nz = nz;
-  e->ndim = (nz > 1) ? 3 : 2;
+  e->nrec = nrec;
   e->N = (long)nx * ny * nz;
-  e->ex = sh.ex;
-  e->ey = sh.ey;
-  e->ez = sh.ez;
-  e->twx = sh.twx;
-  e->twy = sh.twy;
-  e->twz = sh.twz;
-  e->gy = sh.gy;
-  e->gz = sh.gz;
+  e->NT = e->N * nrec;
   if (e->alloc_all()) {
     delete e;
     return nullptr;
@@ -448,7 +483,7 @@ int Engine::alloc_all() {
   SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   SPX_HIP(hipEventCreate(&ev0));
   SPX_HIP(hipEventCreate(&ev1));
-  const size_t c = sizeof(float2) * (size_t)N, f = sizeof(float) * (size_t)N;
+  const size_t c = sizeof(float2) * (size_t)NT, f = sizeof(float) * (size_t)NT;
   SPX_HIP(hipMalloc(&g[0], c));
   SPX_HIP(hipMalloc(&g[1], c));
   SPX_HIP(hipMalloc(&W, c));
@@ -456,20 +491,20 @@ int Engine::alloc_all() {
   SPX_HIP(hipMalloc(&S1, c));
   SPX_HIP(hipMalloc(&ampx, f));
   SPX_HIP(hipMalloc(&absb, f));
-  SPX_HIP(hipMalloc(&flags, (size_t)N));
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
-  max_partials = last_axis_blocks(sh);
-  if (max_partials < kRedBlocks) max_partials = kRedBlocks;
-  SPX_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)max_partials));
-  SPX_HIP(hipMalloc(&red, sizeof(double) * 8));
-  SPX_HIP(hipMalloc(&hist, sizeof(unsigned) * 3 * 2048));
-  SPX_HIP(hipMalloc(&ctl, sizeof(unsigned) * kCtlWords));
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[0], N, make_float2(0.f, 0.f));
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[1], N, make_float2(0.f, 0.f));
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, gp, N, make_float2(0.f, 0.f));
-  SPX_HIP(hipMemsetAsync(flags, 0, (size_t)N, s));
+  SPX_HIP(hipMalloc(&flags, (size_t)NT));
+  int parts = last_axis_tiles_per_rec(sh);
+  if (parts < rec_blocks(N)) parts = rec_blocks(N);
+  SPX_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)parts * nrec));
+  SPX_HIP(hipMalloc(&red, sizeof(double) * 4 * (size_t)nrec));
+  SPX_HIP(hipMalloc(&hist, sizeof(unsigned) * (size_t)kHistStride * nrec));
+  SPX_HIP(hipMalloc(&ctl, sizeof(unsigned) * (size_t)kCtlStride * nrec));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, g[0], NT, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, g[1], NT, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, gp, NT, make_float2(0.f, 0.f));
+  SPX_HIP(hipMemsetAsync(flags, 0, (size_t)NT, s));
   SPX_HIP(hipMemsetAsync(absb, 0, f, s));
-  hipLaunchKernelGGL(k_encode_amp, ew_grid(N), dim3(256), 0, s, absb, flags, ampx, N);
+  SPX_HIP(hipMemsetAsync(ctl, 0, sizeof(unsigned) * (size_t)kCtlStride * nrec, s));
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(NT), dim3(256), 0, s, absb, flags, ampx, NT);
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -477,7 +512,8 @@ int Engine::alloc_all() {
 Engine::~Engine() {
   // teardown: errors here have nowhere to go
   if (s) (void)hipStreamSynchronize(s);
-  void* ptrs[] = {g[0], g[1], W, Wf, gp, S1, ampx, amp_min, amp_max, absb, phased, flags, partials, red, hist, ctl};
+  void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min, amp_max,
+                  absb, phased, flags, partials, red, hist, ctl, alg_rec, beta_rec};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -486,95 +522,114 @@ Engine::~Engine() {
 }
 
 // encoded amplitudes: the raw values pass through `absb` (scratch) once
-int Engine::upload_amplitudes(const float* amp, const uint8_t* fl) {
-  SPX_HIP(hipMemcpyAsync(absb, amp, sizeof(float) * N, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_encode_amp, ew_grid(N), dim3(256), 0, s, absb, flags, ampx, N);
+int Engine::upload_amplitudes(const float* amp, const uint8_t* fl, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  SPX_HIP(hipMemcpyAsync(absb + off, amp, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(flags + off, fl, (size_t)n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(n), dim3(256), 0, s, absb + off, flags + off, ampx + off, n);
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
   return 0;
 }
 
-int Engine::upload_flags(const uint8_t* fl) {
-  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
+int Engine::upload_flags(const uint8_t* fl, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  SPX_HIP(hipMemcpyAsync(flags + off, fl, (size_t)n, hipMemcpyHostToDevice, s));
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
   return 0;
 }
 
-int Engine::download_flags(uint8_t* fl) {
-  SPX_HIP(hipMemcpyAsync(fl, flags, (size_t)N, hipMemcpyDeviceToHost, s));
+int Engine::download_flags(uint8_t* fl, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  SPX_HIP(hipMemcpyAsync(fl, flags + off, (size_t)n, hipMemcpyDeviceToHost, s));
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
 int Engine::upload_margins(const float* lo, const float* hi) {
-  if (!amp_min) SPX_HIP(hipMalloc(&amp_min, sizeof(float) * N));
-  if (!amp_max) SPX_HIP(hipMalloc(&amp_max, sizeof(float) * N));
-  SPX_HIP(hipMemcpyAsync(amp_min, lo, sizeof(float) * N, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipMemcpyAsync(amp_max, hi, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  if (!amp_min) SPX_HIP(hipMalloc(&amp_min, sizeof(float) * NT));
+  if (!amp_max) SPX_HIP(hipMalloc(&amp_max, sizeof(float) * NT));
+  SPX_HIP(hipMemcpyAsync(amp_min, lo, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(amp_max, hi, sizeof(float) * NT, hipMemcpyHostToDevice, s));
   SPX_HIP(hipStreamSynchronize(s));
-  have_margins = true;
   return 0;
 }
 
 int Engine::upload_phased(const float2* ph, const uint8_t* fl) {
-  if (!phased) SPX_HIP(hipMalloc(&phased, sizeof(float2) * N));
-  SPX_HIP(hipMemcpyAsync(phased, ph, sizeof(float2) * N, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)N, hipMemcpyHostToDevice, s));
+  if (!phased) SPX_HIP(hipMalloc(&phased, sizeof(float2) * NT));
+  SPX_HIP(hipMemcpyAsync(phased, ph, sizeof(float2) * NT, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)NT, hipMemcpyHostToDevice, s));
   SPX_HIP(hipStreamSynchronize(s));
-  have_phased = true;
   ++flags_version;
   return 0;
 }
 
-int Engine::upload_model(const float2* model) {
-  SPX_HIP(hipMemcpyAsync(g[cur], model, sizeof(float2) * N, hipMemcpyHostToDevice, s));
+int Engine::upload_model(const float2* model, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  SPX_HIP(hipMemcpyAsync(g[cur] + off, model, sizeof(float2) * n, hipMemcpyHostToDevice, s));
   SPX_HIP(hipStreamSynchronize(s));
   invalidate_R();
   return 0;
 }
 
-int Engine::download(const float2* dev, float2* host) {
-  SPX_HIP(hipMemcpyAsync(host, dev, sizeof(float2) * N, hipMemcpyDeviceToHost, s));
+int Engine::download(const float2* dev, float2* host, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  SPX_HIP(hipMemcpyAsync(host, dev + off, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::set_rec_params(const int* alg, const float* beta) {
+  if (alg) {
+    if (!alg_rec) SPX_HIP(hipMalloc(&alg_rec, sizeof(int) * nrec));
+    SPX_HIP(hipMemcpyAsync(alg_rec, alg, sizeof(int) * nrec, hipMemcpyHostToDevice, s));
+  }
+  if (beta) {
+    if (!beta_rec) SPX_HIP(hipMalloc(&beta_rec, sizeof(float) * nrec));
+    SPX_HIP(hipMemcpyAsync(beta_rec, beta, sizeof(float) * nrec, hipMemcpyHostToDevice, s));
+  }
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
 int Engine::fft_nd(const float2* in, float2* out, int dir) {
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
   enqueue_fft(sh, in, out, dir, s);
   SPX_HIP(hipGetLastError());
   return 0;
 }
 
-int Engine::reduce_partials(int nparts, int nv, double* host_out) {
-  hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(1024), 0, s, partials, nparts, nv, red);
+int Engine::reduce_per_rec(int parts, int nv, double* host_out) {
+  hipLaunchKernelGGL(k_reduce_rec, dim3(nrec), dim3(256), 0, s, partials, parts, nv, red);
   SPX_HIP(hipGetLastError());
-  SPX_HIP(hipMemcpyAsync(host_out, red, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipMemcpyAsync(host_out, red, sizeof(double) * nv * nrec, hipMemcpyDeviceToHost, s));
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
 int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd) {
-  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * NT, hipMemcpyHostToDevice, s));
   const float inv_n = (float)(1.0 / (double)N);
   if (fl & 1 /* SpModelRandomPhases */ || fl & 2 /* SpModelZeroPhases */) {
-    hipLaunchKernelGGL(k_init_phases, ew_grid(N), dim3(256), 0, s, absb, S1, N, sd, (fl & 1) ? 1 : 0);
+    hipLaunchKernelGGL(k_init_phases, ew_grid(NT), dim3(256), 0, s, absb, S1, NT, sd, (fl & 1) ? 1 : 0);
     if (fft_nd(S1, S1, +1)) return -20;
-    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, S1, g[cur], N, inv_n);
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(NT), dim3(256), 0, s, S1, g[cur], NT, inv_n);
   } else if (fl & 4 /* SpModelRandomValues */) {
-    // reference src/phasing.c:666-679: scale so that sum |g|^2 = sum amp^2
-    double sums[2];
-    hipLaunchKernelGGL(k_sumsq_f32, dim3(kRedBlocks), dim3(kRedThreads), 0, s, absb, N, partials);
-    if (reduce_partials(kRedBlocks, 1, &sums[0])) return -20;
-    hipLaunchKernelGGL(k_random_values, ew_grid(N), dim3(256), 0, s, g[cur], N, sd);
-    hipLaunchKernelGGL(k_sumsq_c64, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g[cur], N, partials);
-    if (reduce_partials(kRedBlocks, 1, &sums[1])) return -20;
-    const double mean = sums[0] / (double)N;
-    const float sc = (float)sqrt(mean / sums[1]);
-    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, g[cur], g[cur], N, sc);
+    // reference src/phasing.c:666-679: scale so that sum |g|^2 = sum amp^2 / N
+    const int bpr = rec_blocks(N);
+    std::vector<double> amp2((size_t)nrec), mod2((size_t)nrec);
+    hipLaunchKernelGGL(k_sumsq_f32, dim3(bpr, nrec), dim3(kRedThreads), 0, s, absb, N, partials);
+    if (reduce_per_rec(bpr, 1, amp2.data())) return -20;
+    hipLaunchKernelGGL(k_random_values, ew_grid(NT), dim3(256), 0, s, g[cur], NT, sd);
+    hipLaunchKernelGGL(k_sumsq_c64, dim3(bpr, nrec), dim3(kRedThreads), 0, s, g[cur], N, partials);
+    if (reduce_per_rec(bpr, 1, mod2.data())) return -20;
+    std::vector<float> sc((size_t)nrec);
+    for (int r = 0; r < nrec; ++r) sc[r] = (float)sqrt((amp2[r] / (double)N) / mod2[r]);
+    float* dsc = (float*)partials;  // free again: the reductions above are done
+    SPX_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * nrec, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scale_rec, ew_grid(NT), dim3(256), 0, s, g[cur], N, NT, dsc);
+    SPX_HIP(hipStreamSynchronize(s));
   } else {
     return -3;
   }
@@ -583,19 +638,19 @@ int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd
 }
 
 int Engine::init_model_user(const float2* model, int) {
-  SPX_HIP(hipMemcpyAsync(g[cur], model, sizeof(float2) * N, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(g[cur], model, sizeof(float2) * NT, hipMemcpyHostToDevice, s));
   return 0;
 }
 
 int Engine::finish_model_init(int fl) {
   if (fl & 256 /* SpModelMaskedOutZeroed */) {
     if (fft_nd(g[cur], S1, -1)) return -20;
-    hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(N), dim3(256), 0, s, S1, flags, N);
+    hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(NT), dim3(256), 0, s, S1, flags, NT);
     if (fft_nd(S1, S1, +1)) return -20;
-    hipLaunchKernelGGL(k_scale_copy, ew_grid(N), dim3(256), 0, s, S1, g[cur], N, (float)(1.0 / (double)N));
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(NT), dim3(256), 0, s, S1, g[cur], NT, (float)(1.0 / (double)N));
   }
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, g[cur ^ 1], N, make_float2(0.f, 0.f));
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(N), dim3(256), 0, s, gp, N, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, g[cur ^ 1], NT, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, gp, NT, make_float2(0.f, 0.f));
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   invalidate_R();
@@ -604,29 +659,17 @@ int Engine::finish_model_init(int fl) {
 
 int Engine::ensure_R() {
   if (R_valid) return 0;
-  RowArgs r{};
-  r.in = g[cur];
-  r.out = W;
-  r.tw = twx;
-  r.nlines = ny * nz;
-  r.L = nx;
-  ex->row(kRowFwd, r, s);
+  sh.ex->row(kRowFwd, row_args(sh, g[cur], W), s);
   SPX_HIP(hipGetLastError());
   R_valid = true;
   return 0;
 }
 
 int Engine::ensure_Rf(float gamma1) {
-  if (!Wf) SPX_HIP(hipMalloc(&Wf, sizeof(float2) * N));
+  if (!Wf) SPX_HIP(hipMalloc(&Wf, sizeof(float2) * NT));
   if (Rf_valid && Rf_version == flags_version && Rf_gamma1 == gamma1) return 0;
-  hipLaunchKernelGGL(k_dm_f1, ew_grid(N), dim3(256), 0, s, g[cur], flags, gamma1, Wf, N);
-  RowArgs r{};
-  r.in = Wf;
-  r.out = Wf;
-  r.tw = twx;
-  r.nlines = ny * nz;
-  r.L = nx;
-  ex->row(kRowFwd, r, s);
+  hipLaunchKernelGGL(k_dm_f1, ew_grid(NT), dim3(256), 0, s, g[cur], flags, gamma1, Wf, NT);
+  sh.ex->row(kRowFwd, row_args(sh, Wf, Wf), s);
   SPX_HIP(hipGetLastError());
   Rf_valid = true;
   Rf_version = flags_version;
@@ -635,12 +678,18 @@ int Engine::ensure_Rf(float gamma1) {
 }
 
 int Engine::iterate(const IterParams& p, const float* betas, int n, bool want_gp) {
+  return iterate_impl(p, betas, n, want_gp, nullptr);
+}
+
+// `marks`, when given, receives 2n+1 events: before the Fourier-space passes
+// of iteration i (2i), between them and the real-space pass (2i+1), and after
+// the last pass (2n).
+int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks) {
   if (n <= 0) return 0;
   const bool dm = (p.alg == kDiffMap);
   if (int r = ensure_R()) return r;
   if (dm)
     if (int r = ensure_Rf(p.gamma1)) return r;
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
   ColArgs ca{};
   ca.ampx = ampx;
   ca.amp_min = amp_min;
@@ -656,8 +705,11 @@ int Engine::iterate(const IterParams& p, const float* betas, int n, bool want_gp
   u.W = W;
   u.Wf = Wf;
   u.flags = flags;
-  u.tw = twx;
-  u.nlines = ny * nz;
+  u.tw = sh.twx;
+  u.beta_rec = beta_rec;
+  u.alg_rec = dm ? nullptr : alg_rec;
+  u.nlines = ny * nz * nrec;
+  u.lines_per_rec = ny * nz;
   u.alg = p.alg;
   u.constraints = p.constraints;
   u.gamma1 = p.gamma1;
@@ -665,15 +717,18 @@ int Engine::iterate(const IterParams& p, const float* betas, int n, bool want_gp
   u.scale = (float)(1.0 / (double)N);
   u.nsize = (float)N;
   for (int i = 0; i < n; ++i) {
+    if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
     enqueue_fourier(sh, kColProject, W, W, ca, s);
     if (dm) enqueue_fourier(sh, kColProject, Wf, Wf, caf, s);
+    if (marks) SPX_HIP(hipEventRecord(marks[2 * i + 1], s));
     u.gcur = g[cur];
     u.gnew = g[cur ^ 1];
     u.gp = (want_gp && i == n - 1) ? gp : nullptr;
-    u.beta = betas[i];
-    ex->update(u, dm, s);
+    u.beta = betas ? betas[i] : 0.f;
+    sh.ex->update(u, dm, s);
     cur ^= 1;
   }
+  if (marks) SPX_HIP(hipEventRecord(marks[2 * n], s));
   SPX_HIP(hipGetLastError());
   if (dm) {
     Rf_version = flags_version;
@@ -696,9 +751,39 @@ int Engine::time_iterations(const IterParams& p, const float* betas, int n, floa
   return 0;
 }
 
-int Engine::efourier(double out[4], float2* fout) {
+int Engine::time_passes(const IterParams& p, const float* betas, int n, float* ms_fourier, float* ms_real) {
+  if (n <= 0) return 0;
   if (int r = ensure_R()) return r;
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
+  if (p.alg == kDiffMap)
+    if (int r = ensure_Rf(p.gamma1)) return r;
+  std::vector<hipEvent_t> ev((size_t)(2 * n + 1), nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = -20;
+  if (!rc) rc = iterate_impl(p, betas, n, false, ev.data());
+  if (!rc && hipEventSynchronize(ev[2 * n]) != hipSuccess) rc = -20;
+  double f = 0.0, r = 0.0;
+  for (int i = 0; i < n && !rc; ++i) {
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&b, ev[2 * i + 1], ev[2 * i + 2]) != hipSuccess)
+      rc = -20;
+    f += a;
+    r += b;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc) {
+    set_error("event timing failed");
+    return rc;
+  }
+  *ms_fourier = (float)f;
+  *ms_real = (float)r;
+  return 0;
+}
+
+int Engine::efourier(double* out, float2* fout) {
+  if (int r = ensure_R()) return r;
   ColArgs ca{};
   ca.ampx = ampx;
   ca.partials = partials;
@@ -706,76 +791,49 @@ int Engine::efourier(double out[4], float2* fout) {
   // the 3-D path needs somewhere to put the y-transformed lines: S1
   enqueue_fourier(sh, kColEFourier, W, S1, ca, s);
   SPX_HIP(hipGetLastError());
-  return reduce_partials(last_axis_blocks(sh), 4, out);
+  return reduce_per_rec(last_axis_tiles_per_rec(sh), 4, out);
 }
 
 int Engine::fmodel(float2* fout) {
-  double dummy[4];
-  return efourier(dummy, fout);
+  std::vector<double> dummy((size_t)4 * nrec);
+  return efourier(dummy.data(), fout);
 }
 
-int Engine::ereal(double out[2]) {
-  hipLaunchKernelGGL(k_ereal, dim3(kRedBlocks), dim3(kRedThreads), 0, s, gp, flags, N, partials);
+int Engine::ereal(double* out) {
+  const int bpr = rec_blocks(N);
+  hipLaunchKernelGGL(k_ereal, dim3(bpr, nrec), dim3(kRedThreads), 0, s, gp, flags, N, partials);
   SPX_HIP(hipGetLastError());
-  return reduce_partials(kRedBlocks, 2, out);
+  return reduce_per_rec(bpr, 2, out);
 }
 
 int Engine::support_count(double* out) {
-  hipLaunchKernelGGL(k_count_support, dim3(kRedBlocks), dim3(kRedThreads), 0, s, flags, N, partials);
+  const int bpr = rec_blocks(N);
+  hipLaunchKernelGGL(k_count_support, dim3(bpr, nrec), dim3(kRedThreads), 0, s, flags, N, partials);
   SPX_HIP(hipGetLastError());
-  return reduce_partials(kRedBlocks, 1, out);
+  return reduce_per_rec(bpr, 1, out);
 }
 
 int Engine::model_change(float2* out) {
-  hipLaunchKernelGGL(k_sub, ew_grid(N), dim3(256), 0, s, g[cur], g[cur ^ 1], out, N);
+  hipLaunchKernelGGL(k_sub, ew_grid(NT), dim3(256), 0, s, g[cur], g[cur ^ 1], out, NT);
   SPX_HIP(hipGetLastError());
   return 0;
 }
 
-int Engine::blur(const float2* in, float2* out, float sigma) {
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
-  RowArgs r{};
-  r.in = in;
-  r.out = out;
-  r.tw = twx;
-  r.nlines = ny * nz;
-  r.L = nx;
-  ex->row(kRowFwd, r, s);
-  ColArgs cb{};
-  cb.nx = nx;
-  cb.ny = ny;
-  cb.nz = nz;
-  cb.sigma = sigma;
-  enqueue_fourier(sh, kColBlur, out, out, cb, s);
-  r.in = out;
-  ex->row(kRowInv, r, s);
-  SPX_HIP(hipGetLastError());
-  return 0;
-}
-
-// |blur(|g1|, sigma)| into absb, with the level-1 histogram and the running
-// maximum (reference src/support_update.c:138-141,169-172)
-int Engine::blur_to_abs(float sigma, unsigned* h, unsigned* maxbits) {
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
-  RowArgs r{};
-  r.in = g[cur];
-  r.out = S1;
-  r.tw = twx;
-  r.nlines = ny * nz;
-  r.L = nx;
-  ex->row(kRowAbsFwd, r, s);
+// |blur(|g1|, sigma)| into absb, with the level-1 histograms and the running
+// maxima (reference src/support_update.c:138-141,169-172)
+int Engine::blur_to_abs(float sigma, bool with_hist) {
+  sh.ex->row(kRowAbsFwd, row_args(sh, g[cur], S1), s);
   ColArgs cb{};
   cb.nx = nx;
   cb.ny = ny;
   cb.nz = nz;
   cb.sigma = sigma;
   enqueue_fourier(sh, kColBlur, S1, S1, cb, s);
-  r.in = S1;
-  r.out = nullptr;
+  RowArgs r = row_args(sh, S1, nullptr);
   r.absb = absb;
-  r.hist = h;
-  r.maxbits = maxbits;
-  ex->row(kRowBlurOut, r, s);
+  r.hist = with_hist ? hist : nullptr;
+  r.ctl = ctl;
+  sh.ex->row(kRowBlurOut, r, s);
   SPX_HIP(hipGetLastError());
   return 0;
 }
@@ -785,32 +843,34 @@ int Engine::shrinkwrap_area(float sigma, float area) {
   long k = (long)(int)((float)N * area);
   if (k >= N) k = N - 1;
   if (k < 0) k = 0;
-  hipLaunchKernelGGL(k_fill_u32, dim3(24), dim3(256), 0, s, hist, 3L * 2048, 0u);
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, (unsigned)k);
-  if (int rc = blur_to_abs(sigma, hist, ctl + kCtlMax)) return rc;
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist, 2048, ctl, 1);
-  hipLaunchKernelGGL(k_hist_level, ew_grid(N), dim3(256), 0, s, absb, N, hist + 2048, ctl, 2);
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist + 2048, 2048, ctl, 2);
-  hipLaunchKernelGGL(k_hist_level, ew_grid(N), dim3(256), 0, s, absb, N, hist + 4096, ctl, 3);
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, hist + 4096, 1024, ctl, 3);
-  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 0, 0.f);
+  const int bpr = rec_blocks(N);
+  const long nh = (long)kHistStride * nrec;
+  hipLaunchKernelGGL(k_fill_u32, ew_grid(nh), dim3(256), 0, s, hist, nh, 0u);
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, ctl, nrec, (unsigned)k);
+  if (int rc = blur_to_abs(sigma, true)) return rc;
+  hipLaunchKernelGGL(k_select, dim3(nrec), dim3(1024), 0, s, hist, 1, ctl);
+  hipLaunchKernelGGL(k_hist_level, dim3(bpr, nrec), dim3(256), 0, s, absb, N, hist, ctl, 2);
+  hipLaunchKernelGGL(k_select, dim3(nrec), dim3(1024), 0, s, hist, 2, ctl);
+  hipLaunchKernelGGL(k_hist_level, dim3(bpr, nrec), dim3(256), 0, s, absb, N, hist, ctl, 3);
+  hipLaunchKernelGGL(k_select, dim3(nrec), dim3(1024), 0, s, hist, 3, ctl);
+  hipLaunchKernelGGL(k_mask, dim3(bpr, nrec), dim3(256), 0, s, absb, flags, N, ctl, 0, 0.f);
   SPX_HIP(hipGetLastError());
   ++flags_version;
   return 0;
 }
 
 int Engine::last_threshold(float* t) {
-  unsigned bits = 0;
-  SPX_HIP(hipMemcpyAsync(&bits, ctl + kCtlThresh, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  std::vector<unsigned> c((size_t)kCtlStride * nrec);
+  SPX_HIP(hipMemcpyAsync(c.data(), ctl, sizeof(unsigned) * c.size(), hipMemcpyDeviceToHost, s));
   SPX_HIP(hipStreamSynchronize(s));
-  memcpy(t, &bits, sizeof(float));
+  for (int r = 0; r < nrec; ++r) memcpy(&t[r], &c[(size_t)r * kCtlStride + kCtlThresh], sizeof(float));
   return 0;
 }
 
 int Engine::shrinkwrap_threshold(float sigma, float rel) {
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, 0u);
-  if (int rc = blur_to_abs(sigma, nullptr, ctl + kCtlMax)) return rc;
-  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 1, rel);
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, ctl, nrec, 0u);
+  if (int rc = blur_to_abs(sigma, false)) return rc;
+  hipLaunchKernelGGL(k_mask, dim3(rec_blocks(N), nrec), dim3(256), 0, s, absb, flags, N, ctl, 1, rel);
   SPX_HIP(hipGetLastError());
   ++flags_version;
   return 0;
@@ -819,34 +879,29 @@ int Engine::shrinkwrap_threshold(float sigma, float rel) {
 // support from the thresholded Patterson map |IFFT(amp^2)| > max * value
 // (reference src/phasing.c:744-763)
 int Engine::patterson_support(const float* amp_raw, float value) {
-  Shape sh{nx, ny, nz, ex, ey, ez, twx, twy, twz, gy, gz};
-  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * N, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_amp2, ew_grid(N), dim3(256), 0, s, absb, S1, N);
+  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_amp2, ew_grid(NT), dim3(256), 0, s, absb, S1, NT);
   ColArgs c{};
-  if (ey) {
+  if (sh.ey) {
     c.in = S1;
     c.W = S1;
-    c.tw = twy;
-    c.geo = gy;
-    ey->col(kColInv, c, s);
+    c.tw = sh.twy;
+    c.geo = sh.gy;
+    sh.ey->col(kColInv, c, s);
   }
-  if (ez) {
+  if (sh.ez) {
     c.in = S1;
     c.W = S1;
-    c.tw = twz;
-    c.geo = gz;
-    ez->col(kColInv, c, s);
+    c.tw = sh.twz;
+    c.geo = sh.gz;
+    sh.ez->col(kColInv, c, s);
   }
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, ctl, 0u);
-  RowArgs r{};
-  r.in = S1;
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, ctl, nrec, 0u);
+  RowArgs r = row_args(sh, S1, nullptr);
   r.absb = absb;
-  r.maxbits = ctl + kCtlMax;
-  r.tw = twx;
-  r.nlines = ny * nz;
-  r.L = nx;
-  ex->row(kRowBlurOut, r, s);
-  hipLaunchKernelGGL(k_mask, ew_grid(N), dim3(256), 0, s, absb, flags, N, ctl, 1, value);
+  r.ctl = ctl;
+  sh.ex->row(kRowBlurOut, r, s);
+  hipLaunchKernelGGL(k_mask, dim3(rec_blocks(N), nrec), dim3(256), 0, s, absb, flags, N, ctl, 1, value);
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
@@ -865,7 +920,11 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
     return -10;
   }
   Shape sh;
-  if (int rc = make_shape(nx, ny, nz, &sh)) return rc;
+  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
+  if (is_blur && (!sh.ex || (!sh.ey && !sh.ez))) {
+    set_error("blur needs a 2-D or 3-D array with nx > 1");
+    return -11;
+  }
   static hipStream_t s = nullptr;
   if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   const size_t bytes = sizeof(float2) * (size_t)nx * ny * nz;
@@ -877,30 +936,14 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
     if (!is_blur) {
       enqueue_fft(sh, d, d, dir, s);
     } else {
-      if (!sh.ex) {
-        set_error("blur needs nx > 1");
-        rc = -11;
-      } else {
-        RowArgs r{};
-        r.in = d;
-        r.out = d;
-        r.tw = sh.twx;
-        r.nlines = ny * nz;
-        r.L = nx;
-        sh.ex->row(kRowFwd, r, s);
-        if (sh.ey || sh.ez) {
-          ColArgs cb{};
-          cb.nx = nx;
-          cb.ny = ny;
-          cb.nz = nz;
-          cb.sigma = sigma;
-          enqueue_fourier(sh, kColBlur, d, d, cb, s);
-        } else {
-          rc = -11;
-          set_error("blur needs a 2-D or 3-D array");
-        }
-        sh.ex->row(kRowInv, r, s);
-      }
+      sh.ex->row(kRowFwd, row_args(sh, d, d), s);
+      ColArgs cb{};
+      cb.nx = nx;
+      cb.ny = ny;
+      cb.nz = nz;
+      cb.sigma = sigma;
+      enqueue_fourier(sh, kColBlur, d, d, cb, s);
+      sh.ex->row(kRowInv, row_args(sh, d, d), s);
     }
   }
   if (!rc && hipGetLastError() != hipSuccess) rc = -20;
@@ -926,7 +969,7 @@ int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
     return -10;
   }
   Shape sh;
-  if (int rc = make_shape(nx, ny, nz, &sh)) return rc;
+  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
   static hipStream_t s = nullptr;
   if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   enqueue_fft(sh, in, out, dir, s);

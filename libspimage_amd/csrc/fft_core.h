@@ -1,12 +1,21 @@
-// Device building blocks of the LDS-resident Stockham FFT used by every pass
-// kernel of the phasing engine.
+// Device building blocks of the Stockham FFT used by every pass kernel of the
+// phasing engine.
 //
 // Conventions follow the reference's FFTW3 calls (src/fft.c:79,95,163,185):
 // unnormalised complex-to-complex transforms, forward sign -1
 // (X_k = sum_n x_n e^{-2 pi i k n / L}), inverse sign +1, row-major with x
-// fastest.  A length-L transform is split into radix stages (16, 8, 4, 2, 3, 5)
-// chosen at compile time; each stage reads and writes the line through LDS in
-// Stockham (self-sorting) order, so no bit-reversal pass exists.
+// fastest.
+//
+// A length-L line is split into radix stages (2, 3, 4, 5, 8, 16) chosen at
+// compile time.  T threads cooperate on a line.  Stage 0 reads its inputs from
+// registers (loaded straight from global memory: butterfly b takes the
+// elements b + k L/R0, so consecutive threads read consecutive addresses),
+// the middle stages go through LDS (two buffers, one barrier per stage) and
+// the last stage leaves its outputs in registers at b + r L/R_last.  Plans
+// start and end with the same radix whenever that does not cost an extra
+// stage, so the output positions of one transform are the input positions of
+// the next: an element-wise step between an inverse and a forward transform
+// (projection, algorithm update) never touches LDS.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,14 +33,40 @@ struct FftPlan {
   int twoff[16];  // offset of the stage's twiddles in the per-length table
   int twtotal;    // total twiddle entries (complex) for this length
   int maxr;       // largest radix
-  int T;          // threads cooperating on one line = L / maxr
+  int T;          // threads cooperating on one line
   bool ok;        // L factors into the supported radices
 };
+
+// twiddle row stride of a radix-R stage: R-1 entries padded to an even count
+// so a row can be read as float4 pairs
+constexpr int tw_stride(int R) { return ((R - 1) + 1) & ~1; }
 
 constexpr int ilog2c(int v) {
   int e = 0;
   while ((1 << e) < v) ++e;
   return e;
+}
+constexpr int gcd_c(int a, int b) {
+  while (b) {
+    const int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+constexpr int pow2_stages(int e) { return (e + 3) / 4; }
+
+// Appends the radices of 2^e, largest first (16, 8, 4, 2).
+constexpr void push_pow2(FftPlan& p, int e) {
+  while (e > 0) {
+    int take = 0;
+    if (e >= 8 || e == 4 || e == 7) take = 4;
+    else if (e >= 3) take = 3;
+    else take = e;
+    if (e == 5) take = 3;
+    p.r[p.n++] = 1 << take;
+    e -= take;
+  }
 }
 
 constexpr FftPlan make_plan(int L) {
@@ -39,32 +74,56 @@ constexpr FftPlan make_plan(int L) {
   p.L = L;
   p.n = 0;
   p.ok = L >= 2;
-  int rem = L;
-  while (rem % 5 == 0) { p.r[p.n++] = 5; rem /= 5; }
-  while (rem % 3 == 0) { p.r[p.n++] = 3; rem /= 3; }
-  if ((rem & (rem - 1)) != 0) p.ok = false;
-  int e = ilog2c(rem);
-  // split the power-of-two part into radix 16/8/4/2 stages, largest first,
-  // never leaving a lone radix-2 behind a radix-16 when two radix-8 fit.
-  while (e > 0) {
-    int take = 0;
-    if (e >= 10 || e == 4 || e == 7 || e == 8) take = 4;   // 2^10 = 16*8*8
-    else if (e >= 3) take = 3;                              // 2^9 = 8*8*8, 2^5 = 8*4
-    else take = e;
-    p.r[p.n++] = 1 << take;
-    e -= take;
+  int rem = L, n3 = 0, n5 = 0;
+  while (rem % 5 == 0) { ++n5; rem /= 5; }
+  while (rem % 3 == 0) { ++n3; rem /= 3; }
+  if (!p.ok || (rem & (rem - 1)) != 0) {
+    p.ok = false;
+    return p;
+  }
+  const int a2 = ilog2c(rem);
+  if (L == 2 || L == 3 || L == 4 || L == 5 || L == 8 || L == 16) {
+    p.r[p.n++] = L;
+  } else {
+    // palindromic edges 2^ke ... 2^ke with the fewest stages, larger edges
+    // on ties; a plain plan when that saves a stage even after the LDS
+    // round-trip it costs between chained transforms
+    int best_ke = 0, best = 1 << 20;
+    for (int ke = 4; ke >= 1; --ke) {
+      if (2 * ke > a2) continue;
+      const int st = 2 + pow2_stages(a2 - 2 * ke) + n3 + n5;
+      if (st < best) {
+        best = st;
+        best_ke = ke;
+      }
+    }
+    const int plain = pow2_stages(a2) + n3 + n5;
+    if (best_ke > 0 && best <= plain + 1) {
+      p.r[p.n++] = 1 << best_ke;
+      push_pow2(p, a2 - 2 * best_ke);
+      for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
+      for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
+      p.r[p.n++] = 1 << best_ke;
+    } else {
+      push_pow2(p, a2);
+      for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
+      for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
+    }
   }
   int ns = 1, off = 0, mr = 1;
   for (int s = 0; s < p.n; ++s) {
     p.ns[s] = ns;
     p.twoff[s] = off;
-    off += ns * (p.r[s] - 1);
+    if (s > 0) off += ns * tw_stride(p.r[s]);
     ns *= p.r[s];
     if (p.r[s] > mr) mr = p.r[s];
   }
   p.twtotal = off;
   p.maxr = mr;
-  p.T = L / mr;
+  const int e0 = p.r[0], el = p.r[p.n - 1];
+  const int lcm = e0 / gcd_c(e0, el) * el;
+  const int m = (mr % e0 == 0 && mr % el == 0) ? mr : lcm;
+  p.T = L / m;
   if (ns != L) p.ok = false;
   return p;
 }
@@ -74,6 +133,13 @@ struct PlanOf {
   static constexpr FftPlan P = make_plan(L);
   static_assert(P.ok, "transform length does not factor into radices 2,3,4,5,8,16");
   static constexpr int T = P.T;
+  static constexpr int E0 = P.r[0];
+  static constexpr int EL = P.r[P.n - 1];
+  static constexpr int NREG = L / T;     // elements each thread holds at the edges
+  static constexpr int B0 = (L / E0) / T;  // stage-0 butterflies per thread
+  static constexpr int BL = (L / EL) / T;  // last-stage butterflies per thread
+  static constexpr bool PAL = (E0 == EL);  // output positions == input positions
+  static_assert(B0 * E0 == NREG && BL * EL == NREG, "edge stages must divide evenly over the threads");
 };
 
 // ---------------------------------------------------------------------------
@@ -81,7 +147,6 @@ struct PlanOf {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cscl(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -101,15 +166,15 @@ __device__ __forceinline__ float2 twmul(float2 a, float2 w) {
 // ---------------------------------------------------------------------------
 template <int DIR>
 __device__ __forceinline__ void dft2(float2& a, float2& b) {
-  float2 t = a;
+  const float2 t = a;
   a = cadd(t, b);
   b = csub(t, b);
 }
 
 template <int DIR>
 __device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3) {
-  float2 t0 = cadd(x0, x2), t1 = csub(x0, x2);
-  float2 t2 = cadd(x1, x3), t3 = rot<DIR>(csub(x1, x3));
+  const float2 t0 = cadd(x0, x2), t1 = csub(x0, x2);
+  const float2 t2 = cadd(x1, x3), t3 = rot<DIR>(csub(x1, x3));
   x0 = cadd(t0, t2);
   x2 = csub(t0, t2);
   x1 = cadd(t1, t3);
@@ -124,9 +189,9 @@ __device__ __forceinline__ void dft8(float2* x) {
   dft4<DIR>(e0, e1, e2, e3);
   dft4<DIR>(o0, o1, o2, o3);
   // W8^1 = (c, DIR c), W8^2 = DIR i, W8^3 = (-c, DIR c)
-  float2 w1 = make_float2(c * (o1.x - DIR * o1.y), c * (o1.y + DIR * o1.x));
-  float2 w2 = rot<DIR>(o2);
-  float2 w3 = make_float2(c * (-o3.x - DIR * o3.y), c * (-o3.y + DIR * o3.x));
+  const float2 w1 = make_float2(c * (o1.x - DIR * o1.y), c * (o1.y + DIR * o1.x));
+  const float2 w2 = rot<DIR>(o2);
+  const float2 w3 = make_float2(c * (-o3.x - DIR * o3.y), c * (-o3.y + DIR * o3.x));
   x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
   x[1] = cadd(e1, w1); x[5] = csub(e1, w1);
   x[2] = cadd(e2, w2); x[6] = csub(e2, w2);
@@ -159,9 +224,9 @@ __device__ __forceinline__ void dft16(float2* x) {
 template <int DIR>
 __device__ __forceinline__ void dft3(float2* x) {
   constexpr float h = 0.86602540378443864676f;  // sqrt(3)/2
-  float2 s = cadd(x[1], x[2]);
-  float2 d = rot<DIR>(csub(x[1], x[2]));
-  float2 m = make_float2(x[0].x - 0.5f * s.x, x[0].y - 0.5f * s.y);
+  const float2 s = cadd(x[1], x[2]);
+  const float2 d = rot<DIR>(csub(x[1], x[2]));
+  const float2 m = make_float2(x[0].x - 0.5f * s.x, x[0].y - 0.5f * s.y);
   x[0] = cadd(x[0], s);
   x[1] = make_float2(m.x + h * d.x, m.y + h * d.y);
   x[2] = make_float2(m.x - h * d.x, m.y - h * d.y);
@@ -173,13 +238,13 @@ __device__ __forceinline__ void dft5(float2* x) {
   constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
   constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
   constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
-  float2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
-  float2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
-  float2 x0 = x[0];
-  float2 m1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
-  float2 m2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
-  float2 n1 = rot<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
-  float2 n2 = rot<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  const float2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+  const float2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+  const float2 x0 = x[0];
+  const float2 m1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  const float2 m2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  const float2 n1 = rot<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  const float2 n2 = rot<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
   x[0] = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
   x[1] = cadd(m1, n1);
   x[4] = csub(m1, n1);
@@ -216,57 +281,132 @@ struct LdsView {
   }
 };
 
-// One Stockham radix-R stage over a line held in LDS.  `t` is the thread's
-// index among the T threads of its line.  Reads happen before the barrier,
-// writes after it, so the stage runs in place.
-template <int L, int R, int NS, int T, int DIR, int TWOFF, class View>
-__device__ __forceinline__ void stockham_stage(const View& v, int t, const float2* __restrict__ tw) {
+// Twiddles of butterfly j of a stage (forward sense), R-1 values.
+template <int R>
+__device__ __forceinline__ void load_twiddles(const float2* __restrict__ row, float2 (&w)[R]) {
+  constexpr int S = tw_stride(R);
+  static_assert(S % 2 == 0, "twiddle rows are float4 aligned");
+  const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+  for (int q = 0; q < S / 2; ++q) {
+    const float4 v = r4[q];
+    if (2 * q + 1 < R) w[2 * q + 1] = make_float2(v.x, v.y);
+    if (2 * q + 2 < R) w[2 * q + 2] = make_float2(v.z, v.w);
+  }
+}
+
+// Middle stage S: LDS `in` -> LDS `out` (no barrier inside).
+template <int L, int S, int DIR, class View>
+__device__ __forceinline__ void mid_stage(const View& in, const View& out, int t, const float2* __restrict__ tw) {
+  constexpr FftPlan P = PlanOf<L>::P;
+  constexpr int R = P.r[S], NS = P.ns[S], T = P.T;
   constexpr int NB = L / R;
   constexpr int BPT = (NB + T - 1) / T;
-  float2 x[BPT][R];
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int b = t + i * T;
     if (BPT * T == NB || b < NB) {
+      float2 x[R];
 #pragma unroll
-      for (int k = 0; k < R; ++k) x[i][k] = v.at(b + k * NB);
-    }
-  }
-  __syncthreads();
+      for (int k = 0; k < R; ++k) x[k] = in.at(b + k * NB);
+      const int j = b % NS;
+      float2 w[R];
+      load_twiddles<R>(tw + P.twoff[S] + j * tw_stride(R), w);
 #pragma unroll
-  for (int i = 0; i < BPT; ++i) {
-    const int b = t + i * T;
-    if (BPT * T == NB || b < NB) {
-      const int j = (NS == 1) ? 0 : (b % NS);
-      if constexpr (NS > 1) {
-        const float2* w = tw + TWOFF + j * (R - 1);
-#pragma unroll
-        for (int k = 1; k < R; ++k) x[i][k] = twmul<DIR>(x[i][k], w[k - 1]);
-      }
-      dft<R, DIR>(x[i]);
+      for (int k = 1; k < R; ++k) x[k] = twmul<DIR>(x[k], w[k]);
+      dft<R, DIR>(x);
       const int base = (b - j) * R + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v.at(base + r * NS) = x[i][r];
+      for (int r = 0; r < R; ++r) out.at(base + r * NS) = x[r];
     }
   }
-  __syncthreads();
 }
 
-template <int L, int DIR, int S, class View>
-__device__ __forceinline__ void run_stages(const View& v, int t, const float2* __restrict__ tw) {
+template <int L, int S, int DIR, class View>
+__device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, const float2* __restrict__ tw) {
   constexpr FftPlan P = PlanOf<L>::P;
-  if constexpr (S < P.n) {
-    stockham_stage<L, P.r[S], P.ns[S], P.T, DIR, P.twoff[S]>(v, t, tw);
-    run_stages<L, DIR, S + 1>(v, t, tw);
+  if constexpr (S < P.n - 1) {
+    // stage S reads the buffer stage S-1 wrote: a for odd S, b for even S
+    if constexpr (S % 2 == 1) mid_stage<L, S, DIR>(a, b, t, tw);
+    else mid_stage<L, S, DIR>(b, a, t, tw);
+    __syncthreads();
+    mid_stages<L, S + 1, DIR>(a, b, t, tw);
   }
 }
 
-// Full length-L transform of the line in LDS (natural order in, natural order
-// out).  Caller must have synchronised after filling LDS; the last stage ends
-// with a barrier so the result may be read immediately.
+// Full transform of one line.  On entry v[i*E0 + k] holds the element at
+// in_pos(t, i, k) = (t + i T) + k L/E0; on exit v[i*EL + r] holds the output
+// at out_pos(t, i, r) = (t + i T) + r L/EL.  Uses LDS buffers a and b; ends
+// without a barrier (the last stage only reads LDS), so a caller that writes
+// LDS next must synchronise first.
 template <int L, int DIR, class View>
-__device__ __forceinline__ void fft_line(const View& v, int t, const float2* __restrict__ tw) {
-  run_stages<L, DIR, 0>(v, t, tw);
+__device__ __forceinline__ void fft_line(float2 (&v)[PlanOf<L>::NREG], const View& a, const View& b, int t,
+                                         const float2* __restrict__ tw) {
+  constexpr FftPlan P = PlanOf<L>::P;
+  constexpr int T = P.T;
+  if constexpr (P.n == 1) {
+    dft<L, DIR>(v);
+  } else {
+    constexpr int E0 = P.r[0], B0 = PlanOf<L>::B0;
+    // stage 0: no twiddles (Ns = 1); outputs land at b*E0 + r
+#pragma unroll
+    for (int i = 0; i < B0; ++i) {
+      dft<E0, DIR>(v + i * E0);
+      const int bb = t + i * T;
+#pragma unroll
+      for (int r = 0; r < E0; ++r) a.at(bb * E0 + r) = v[i * E0 + r];
+    }
+    __syncthreads();
+    mid_stages<L, 1, DIR>(a, b, t, tw);
+    // last stage reads the buffer written by stage n-2
+    constexpr int SL = P.n - 1;
+    constexpr int EL = P.r[SL], NS = P.ns[SL], BL = PlanOf<L>::BL;
+    const View& src = (SL % 2 == 1) ? a : b;
+#pragma unroll
+    for (int i = 0; i < BL; ++i) {
+      const int bb = t + i * T;  // bb < NS: j = bb
+      float2* x = v + i * EL;
+#pragma unroll
+      for (int k = 0; k < EL; ++k) x[k] = src.at(bb + k * NS);
+      float2 w[EL];
+      load_twiddles<EL>(tw + P.twoff[SL] + bb * tw_stride(EL), w);
+#pragma unroll
+      for (int k = 1; k < EL; ++k) x[k] = twmul<DIR>(x[k], w[k]);
+      dft<EL, DIR>(x);
+    }
+  }
+}
+
+// Moves the outputs of one transform to the input positions of the next
+// when the plan's edge radices differ (a no-op for palindromic plans).
+template <int L, class View>
+__device__ __forceinline__ void out_to_in(float2 (&v)[PlanOf<L>::NREG], const View& a, int t) {
+  if constexpr (!PlanOf<L>::PAL) {
+    constexpr int T = PlanOf<L>::T, E0 = PlanOf<L>::E0, EL = PlanOf<L>::EL;
+    constexpr int B0 = PlanOf<L>::B0, BL = PlanOf<L>::BL;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BL; ++i)
+#pragma unroll
+      for (int r = 0; r < EL; ++r) a.at((t + i * T) + r * (L / EL)) = v[i * EL + r];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B0; ++i)
+#pragma unroll
+      for (int k = 0; k < E0; ++k) v[i * E0 + k] = a.at((t + i * T) + k * (L / E0));
+  }
+}
+
+// positions of the edge registers
+template <int L>
+__device__ __forceinline__ int in_pos(int t, int q) {
+  constexpr int E0 = PlanOf<L>::E0, T = PlanOf<L>::T;
+  return (t + (q / E0) * T) + (q % E0) * (L / E0);
+}
+template <int L>
+__device__ __forceinline__ int out_pos(int t, int q) {
+  constexpr int EL = PlanOf<L>::EL, T = PlanOf<L>::T;
+  return (t + (q / EL) * T) + (q % EL) * (L / EL);
 }
 
 }  // namespace spx

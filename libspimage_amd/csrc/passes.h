@@ -68,14 +68,21 @@ enum RowMode : int {
   kRowBlurOut = 3,    // absb = |IFFT_x(in)| plus max / level-1 histogram
 };
 
+// Per-reconstruction select scratch: 3 histograms of 2048 bins and a small
+// control block (k, prefix, threshold bits, max bits) for each reconstruction.
+constexpr int kHistStride = 3 * 2048;
+constexpr int kCtlStride = 8;
+enum CtlSlot { kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlMax = 3 };
+
 struct RowArgs {
   const float2* in;
   float2* out;
   float* absb;             // kRowBlurOut: |blur| output (f32)
-  unsigned* hist;          // kRowBlurOut: level-1 histogram (2048 bins), or null
-  unsigned* maxbits;       // kRowBlurOut: running max of |blur| as bits, or null
+  unsigned* hist;          // kRowBlurOut: per-reconstruction histograms, or null
+  unsigned* ctl;           // kRowBlurOut: per-reconstruction control (running max), or null
   const float2* tw;
   int nlines;
+  int lines_per_rec;       // lines of one reconstruction (ny * nz)
   int L;
 };
 
@@ -87,7 +94,10 @@ struct UpdateArgs {
   float2* gp;              // model before projection, or null
   const uint8_t* flags;
   const float2* tw;
+  const float* beta_rec;   // per-reconstruction beta, or null for `beta`
+  const int* alg_rec;      // per-reconstruction algorithm, or null for `alg`
   int nlines;
+  int lines_per_rec;
   int alg;
   int constraints;
   float beta;
@@ -195,104 +205,132 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV
 }
 
 // ---------------------------------------------------------------------------
-// Row kernels.
+// Row kernels: M lines along x per work-group, T threads per line.
 // ---------------------------------------------------------------------------
 template <int L>
 struct RowShape {
   static constexpr int T = PlanOf<L>::T;
-  static constexpr int M = (T >= 256) ? 1 : (256 / T);
+  static constexpr int M = (T >= 64) ? 1 : (64 / T);
   static constexpr int NT = T * M;
 };
 
 template <int L, int MODE>
 __global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
   constexpr int T = RowShape<L>::T, M = RowShape<L>::M, NT = RowShape<L>::NT;
+  constexpr int NR = PlanOf<L>::NREG;
   using View = LdsView<L, M, false>;
-  __shared__ float2 sh[View::SIZE];
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[View::SIZE];
   __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
   const bool act = line < a.nlines;
-  const View v{sh, g};
+  const View va{shA, g}, vb{shB, g};
   const long base = (long)line * L;
 
   if constexpr (MODE == kRowBlurOut) {
     if (a.hist)
       for (int i = tid; i < 2048; i += NT) hsh[i] = 0u;
   }
-#pragma unroll 4
-  for (int pos = t; pos < L; pos += T) {
+  float2 v[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
     float2 x = make_float2(0.f, 0.f);
     if (act) {
-      x = a.in[base + pos];
+      x = a.in[base + in_pos<L>(t, q)];
       if constexpr (MODE == kRowAbsFwd) x = make_float2(cabsf_ref(x), 0.f);
     }
-    v.at(pos) = x;
+    v[q] = x;
   }
-  __syncthreads();
-  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, +1>(v, t, a.tw);
-  else fft_line<L, -1>(v, t, a.tw);
+  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, +1>(v, va, vb, t, a.tw);
+  else fft_line<L, -1>(v, va, vb, t, a.tw);
 
   if constexpr (MODE == kRowBlurOut) {
+    // the work-group's lines belong to one reconstruction unless lines of
+    // several small reconstructions share it; then counts go straight to
+    // global memory
+    const int rec = line / a.lines_per_rec;
+    const int rec_first = (blk * M) / a.lines_per_rec;
+    const int last_line = (blk * M + M - 1) < a.nlines ? (blk * M + M - 1) : (a.nlines - 1);
+    const bool one_rec = rec_first == last_line / a.lines_per_rec;
     float mx = 0.f;
-    for (int pos = t; pos < L; pos += T) {
+    if (a.hist) __syncthreads();  // histogram zeroing done
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       if (act) {
-        const float m = cabsf_ref(v.at(pos));
-        a.absb[base + pos] = m;
+        const float m = cabsf_ref(v[q]);
+        a.absb[base + out_pos<L>(t, q)] = m;
         mx = fmaxf(mx, m);
-        if (a.hist) atomicAdd(&hsh[__float_as_uint(m) >> 21], 1u);
+        if (a.hist) {
+          if (one_rec) atomicAdd(&hsh[__float_as_uint(m) >> 21], 1u);
+          else atomicAdd(&a.hist[(long)rec * kHistStride + (__float_as_uint(m) >> 21)], 1u);
+        }
       }
     }
-    if (a.maxbits) {
+    if (a.ctl) {
+      if (one_rec) {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-      if ((tid & 63) == 0) atomicMax(a.maxbits, __float_as_uint(mx));
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        if ((tid & 63) == 0) atomicMax(&a.ctl[rec_first * kCtlStride + kCtlMax], __float_as_uint(mx));
+      } else if (act) {
+        atomicMax(&a.ctl[rec * kCtlStride + kCtlMax], __float_as_uint(mx));
+      }
     }
-    if (a.hist) {
+    if (a.hist && one_rec) {
       __syncthreads();
+      unsigned* h = a.hist + (long)rec_first * kHistStride;
       for (int i = tid; i < 2048; i += NT)
-        if (hsh[i]) atomicAdd(&a.hist[i], hsh[i]);
+        if (hsh[i]) atomicAdd(&h[i], hsh[i]);
     }
   } else {
-    for (int pos = t; pos < L; pos += T)
-      if (act) a.out[base + pos] = v.at(pos);
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      if (act) a.out[base + out_pos<L>(t, q)] = v[q];
   }
 }
 
 // The fused real-space pass of one iteration: inverse x-transform of the
 // column-pass output, 1/N scale, algorithm update, support constraints, store
 // of the new iterate, then forward x-transform of the new iterate (and, for
-// the difference map, of its f1) for the next iteration's column pass.
+// the difference map, of its f1) for the next iteration's column pass.  The
+// update runs on registers at the inverse transform's output positions.
 template <int L, bool DM>
 __global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
   constexpr int T = RowShape<L>::T, M = RowShape<L>::M;
+  constexpr int NR = PlanOf<L>::NREG;
   using View = LdsView<L, M, false>;
-  __shared__ float2 sh[View::SIZE];
-  __shared__ float2 shf[DM ? View::SIZE : 1];
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[View::SIZE];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
   const bool act = line < a.nlines;
-  const View v{sh, g};
-  const View vf{shf, g};
+  const View va{shA, g}, vb{shB, g};
   const long base = (long)line * L;
 
-#pragma unroll 4
-  for (int pos = t; pos < L; pos += T) {
-    v.at(pos) = act ? a.W[base + pos] : make_float2(0.f, 0.f);
-    if constexpr (DM) vf.at(pos) = act ? a.Wf[base + pos] : make_float2(0.f, 0.f);
+  float2 v[NR], vf[DM ? NR : 1];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    v[q] = act ? a.W[base + in_pos<L>(t, q)] : make_float2(0.f, 0.f);
+    if constexpr (DM) vf[q] = act ? a.Wf[base + in_pos<L>(t, q)] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  fft_line<L, +1>(v, t, a.tw);
-  if constexpr (DM) fft_line<L, +1>(vf, t, a.tw);
+  fft_line<L, +1>(v, va, vb, t, a.tw);
+  if constexpr (DM) {
+    __syncthreads();
+    fft_line<L, +1>(vf, va, vb, t, a.tw);
+  }
 
-  for (int pos = t; pos < L; pos += T) {
+  const int rec = act ? line / a.lines_per_rec : 0;
+  const float beta = a.beta_rec ? a.beta_rec[rec] : a.beta;
+  const int alg = a.alg_rec ? a.alg_rec[rec] : a.alg;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
     if (!act) continue;
-    const long idx = base + pos;
-    const float2 P = v.at(pos);
+    const long idx = base + out_pos<L>(t, q);
+    const float2 P = v[q];
     const float2 g0 = a.gcur[idx];
     const bool in = a.flags[idx] & kInsideSupport;
     float2 g1;
@@ -301,65 +339,69 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
       if (a.gp) a.gp[idx] = gp;
       if (in) {
         g1 = gp;
-      } else if (a.alg == kHIO) {
-        g1 = make_float2(g0.x - gp.x * a.beta, g0.y - gp.y * a.beta);
-      } else if (a.alg == kRAAR) {
-        const float c1 = 1.f - 2.f * a.beta;
-        g1 = make_float2(gp.x * c1 + g0.x * a.beta, gp.y * c1 + g0.y * a.beta);
+      } else if (alg == kHIO) {
+        g1 = make_float2(g0.x - gp.x * beta, g0.y - gp.y * beta);
+      } else if (alg == kRAAR) {
+        const float c1 = 1.f - 2.f * beta;
+        g1 = make_float2(gp.x * c1 + g0.x * beta, gp.y * c1 + g0.y * beta);
       } else {
         g1 = make_float2(0.f, 0.f);
       }
     } else {
       // src/phasing.c:1243-1252; gp stays unscaled like the CPU engine
       if (a.gp) a.gp[idx] = P;
-      const float2 Pf = vf.at(pos);
+      const float2 Pf = vf[q];
       if (in) {
-        g1.x = g0.x + a.beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x);
-        g1.y = g0.y + a.beta * ((1.f + a.gamma2) * P.y / a.nsize - a.gamma2 * g0.y);
+        g1.x = g0.x + beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x);
+        g1.y = g0.y + beta * ((1.f + a.gamma2) * P.y / a.nsize - a.gamma2 * g0.y);
       } else {
         g1 = g0;
       }
-      g1.x -= a.beta * Pf.x / a.nsize;
-      g1.y -= a.beta * Pf.y / a.nsize;
+      g1.x -= beta * Pf.x / a.nsize;
+      g1.y -= beta * Pf.y / a.nsize;
     }
     if (in) g1 = apply_constraints(g1, a.constraints);
     a.gnew[idx] = g1;
-    v.at(pos) = g1;
-    if constexpr (DM) {
-      // f1 of the next iteration (src/phasing.c:1260-1274)
-      vf.at(pos) = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
-    }
+    v[q] = g1;
+    // f1 of the next iteration (src/phasing.c:1260-1274)
+    if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
   }
+  out_to_in<L>(v, va, t);
+  if constexpr (DM) out_to_in<L>(vf, va, t);
   __syncthreads();
-  fft_line<L, -1>(v, t, a.tw);
-  if constexpr (DM) fft_line<L, -1>(vf, t, a.tw);
-  for (int pos = t; pos < L; pos += T) {
+  fft_line<L, -1>(v, va, vb, t, a.tw);
+  if constexpr (DM) {
+    __syncthreads();
+    fft_line<L, -1>(vf, va, vb, t, a.tw);
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
     if (!act) continue;
-    a.W[base + pos] = v.at(pos);
-    if constexpr (DM) a.Wf[base + pos] = vf.at(pos);
+    a.W[base + out_pos<L>(t, q)] = v[q];
+    if constexpr (DM) a.Wf[base + out_pos<L>(t, q)] = vf[q];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Column kernels.
+// Column kernels: C adjacent lines (contiguous in memory) per work-group.
 // ---------------------------------------------------------------------------
 template <int L>
 struct ColShape {
   static constexpr int T = PlanOf<L>::T;
-  // adjacent lines per work-group: 8 (64 contiguous bytes) for long lines,
-  // more for short ones so that a work-group still has >= 128 threads
-  // (at most 64 KiB of LDS: 8192 complex values)
-  static constexpr int Cmem = (8192 / L) >= 1 ? (8192 / L) : 1;
-  static constexpr int C0 = ((L >= 256) ? 8 : 16) < Cmem ? ((L >= 256) ? 8 : 16) : Cmem;
-  static constexpr int C = (T * C0 >= 128 || L * C0 * 2 > 8192) ? C0 : ((128 / T) < 64 ? (128 / T) : 64);
+  // two LDS buffers of L*C complex values within 64 KiB
+  static constexpr int Cmem = (4096 / L) >= 1 ? (4096 / L) : 1;
+  static constexpr int C1 = Cmem < 16 ? Cmem : 16;
+  static constexpr int C = (T * C1 >= 64) ? C1 : ((64 / T) < Cmem ? (64 / T) : Cmem);
   static constexpr int NT = T * C;
 };
 
 template <int L, int MODE>
 __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
   constexpr int T = ColShape<L>::T, C = ColShape<L>::C, NT = ColShape<L>::NT;
+  constexpr int NR = PlanOf<L>::NREG;
   using View = LdsView<L, C, true>;
-  __shared__ float2 sh[View::SIZE];
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[View::SIZE];
   __shared__ double red[(MODE == kColEFourier) ? 4 * 32 : 1];
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
@@ -370,62 +412,66 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
   const bool act = x < a.geo.ncontig;
   const long base = (long)outer * a.geo.outer_stride + x;
   const long st = a.geo.axis_stride;
-  const View v{sh, c};
+  const View va{shA, c}, vb{shB, c};
 
-#pragma unroll 4
-  for (int pos = t; pos < L; pos += T) v.at(pos) = act ? a.in[base + pos * st] : make_float2(0.f, 0.f);
-  __syncthreads();
-  if constexpr (MODE == kColInv) fft_line<L, +1>(v, t, a.tw);
-  else fft_line<L, -1>(v, t, a.tw);
+  float2 v[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = act ? a.in[base + in_pos<L>(t, q) * st] : make_float2(0.f, 0.f);
+  if constexpr (MODE == kColInv) fft_line<L, +1>(v, va, vb, t, a.tw);
+  else fft_line<L, -1>(v, va, vb, t, a.tw);
 
   if constexpr (MODE == kColProject) {
-    for (int pos = t; pos < L; pos += T) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + pos * st;
-      float2 G = v.at(pos);
+      const long idx = base + out_pos<L>(t, q) * st;
+      float2 G = v[q];
       if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
       if (a.objective == 0) {
         G = project_pixel(G, a.ampx[idx], a, idx);
       } else if (a.flags[idx] & kMeasured) {
         G = a.phased_amp[idx];  // src/phasing.c:1103-1109
       }
-      v.at(pos) = G;
+      v[q] = G;
     }
+    out_to_in<L>(v, va, t);
     __syncthreads();
-    fft_line<L, +1>(v, t, a.tw);
+    fft_line<L, +1>(v, va, vb, t, a.tw);
   } else if constexpr (MODE == kColBlur) {
     // Gaussian transfer function of sp_gaussian_blur (src/image_filter.c:80-91)
     // with dist_to_top_left (src/image_util.c:1012-1032), computed in double
     // and rounded to float like the reference.
     const double NN = (double)((float)((long)a.nx * a.ny * a.nz));
-    const double k0 = -2.0 * 9.8696044010893586188;  // -2 * pi^2
+    const double k0 = -2.0 * (3.14159265358979323846 * 3.14159265358979323846);
     const double s2 = (double)a.sigma * (double)a.sigma;
-    for (int pos = t; pos < L; pos += T) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + pos * st;
+      const long idx = base + out_pos<L>(t, q) * st;
       const int ix = (int)(idx % a.nx);
       const long rest = idx / a.nx;
       const int iy = (int)(rest % a.ny);
-      const int iz = (int)(rest / a.ny);
+      const int iz = (int)((rest / a.ny) % a.nz);  // reconstructions stack along z
       const float dx = (a.nx - ix < ix) ? (float)(a.nx - ix) : (float)ix;
       const float dy = (a.ny - iy < iy) ? (float)(a.ny - iy) : (float)iy;
       const float dz = (a.nz - iz < iz) ? (float)(a.nz - iz) : (float)iz;
       const float dist = (float)sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
       const float r = dist / (float)a.nx;
       const float h = (float)(exp(k0 * ((double)r * (double)r) * s2) / NN);
-      const float2 F = v.at(pos);
-      v.at(pos) = make_float2(F.x * h, F.y * h);
+      v[q] = make_float2(v[q].x * h, v[q].y * h);
     }
+    out_to_in<L>(v, va, t);
     __syncthreads();
-    fft_line<L, +1>(v, t, a.tw);
+    fft_line<L, +1>(v, va, vb, t, a.tw);
   } else if constexpr (MODE == kColEFourier) {
     // src/phasing.c:879-888: measured pixels add (|F|-|amp|)^2 and amp^2,
     // unmeasured ones |F|^2; FcFo sums |amp| and |F| over measured pixels.
     double acc[4] = {0.0, 0.0, 0.0, 0.0};  // nom, den, fcfo_num, fcfo_den
-    for (int pos = t; pos < L; pos += T) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + pos * st;
-      const float2 F = v.at(pos);
+      const long idx = base + out_pos<L>(t, q) * st;
+      const float2 F = v[q];
       if (a.fout) a.fout[idx] = F;
       const float af = cabsf_ref(F);
       const float ab = a.ampx[idx];
@@ -441,15 +487,17 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
       }
     }
     block_sum<4, NT>(acc, red);
+    // indexed by tile: the tiles of one reconstruction are contiguous
     if (tid == 0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a.partials[(long)blockIdx.x * 4 + q] = acc[q];
+      for (int qq = 0; qq < 4; ++qq) a.partials[(long)blk * 4 + qq] = acc[qq];
     }
     return;
   }
 
-  for (int pos = t; pos < L; pos += T)
-    if (act) a.W[base + pos * st] = v.at(pos);
+#pragma unroll
+  for (int q = 0; q < NR; ++q)
+    if (act) a.W[base + out_pos<L>(t, q) * st] = v[q];
 }
 
 // ---------------------------------------------------------------------------

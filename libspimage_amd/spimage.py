@@ -239,6 +239,7 @@ _bind("spx_last_error", C.c_char_p)
 _bind("spx_supported_length", C.c_int, C.c_int)
 _bind("spx_fft_device", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float))
+_bind("spx_phaser_time_passes", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _bind("spx_phaser_device_arrays", C.c_int, PPhaser, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
       C.POINTER(C.c_void_p))
 _bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)

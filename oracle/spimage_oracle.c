@@ -512,7 +512,7 @@ static int cmp_desc(const void *a, const void *b) {
 /* blur of |g1| and its magnitudes */
 static R *blur_abs(OrPhaser *p, R radius) {
   const long N = p->N;
-  oc *tmp = (oc *)malloc(sizeof(oc) * N);
+  oc *tmp = (oc *)calloc(N, sizeof(oc));
   for (long i = 0; i < N; ++i) {
     tmp[i].re = cabs_ref(p->g1[i]);
     tmp[i].im = 0;
