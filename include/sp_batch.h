@@ -1,0 +1,74 @@
+/* Batched phasing: many independent reconstructions of one array shape
+ * resident on the GPU and advanced together, one launch per pass for the
+ * whole batch.
+ *
+ * This has no counterpart in the reference, whose Reconstructor runs random
+ * restarts one after another (reference src/_spimage_reconstructor.py:677-740,
+ * reconstruct_loop).  Every reconstruction of a batch follows the same
+ * arithmetic as a single phaser (sp_phaser_iterate, reference
+ * src/phasing.c:895-1258): a batch of one gives bit-identical iterates.
+ *
+ * Reconstruction r of a batch of nrec holds its own amplitudes, measured
+ * mask, support and iterates.  The phasing algorithm and the support
+ * schedule (an SpSupportArray of area / threshold / static updates, with the
+ * reference's update_period semantics) apply to all reconstructions; ER, HIO
+ * and RAAR may be mixed per reconstruction with per-reconstruction beta.
+ *
+ * All functions return 0 on success and a negative code on error
+ * (spx_last_error() holds the message).  Host arrays are in the reference's
+ * layout: (nz, ny, nx), x fastest; complex values are interleaved float pairs.
+ */
+#ifndef SPX_SP_BATCH_H
+#define SPX_SP_BATCH_H 1
+
+#include "sp_phasing.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct SpxBatch SpxBatch;
+
+spimage_EXPORT SpxBatch *spx_batch_create(int nx, int ny, int nz, int nrec);
+spimage_EXPORT void spx_batch_free(SpxBatch *b);
+spimage_EXPORT int spx_batch_size(const SpxBatch *b);
+
+/* inputs of reconstruction `rec` (rec < 0: all, arrays of nrec * N) */
+spimage_EXPORT int spx_batch_set_amplitudes(SpxBatch *b, int rec, const float *amplitudes, const int *mask);
+spimage_EXPORT int spx_batch_set_support(SpxBatch *b, int rec, const float *support);
+spimage_EXPORT int spx_batch_set_model(SpxBatch *b, int rec, const float *model_complex);
+/* SpModelRandomPhases / SpModelZeroPhases / SpModelRandomValues start for all
+ * reconstructions from their amplitudes (reference src/phasing.c:626-679);
+ * the generator stream of reconstruction r differs from every other one */
+spimage_EXPORT int spx_batch_init_model(SpxBatch *b, int flags, unsigned long long seed);
+
+/* algorithm of all reconstructions (the reference's allocators build it) */
+spimage_EXPORT int spx_batch_set_algorithm(SpxBatch *b, SpPhasingAlgorithm *alg);
+/* per-reconstruction ER/HIO/RAAR and beta (overrides the algorithm's type
+ * and beta schedule; NULL restores them) */
+spimage_EXPORT int spx_batch_set_rec_algorithms(SpxBatch *b, const int *alg_types, const float *betas);
+spimage_EXPORT int spx_batch_set_support_schedule(SpxBatch *b, SpSupportArray *sup);
+/* replay each iteration from a captured HIP graph (1) or launch kernels (0) */
+spimage_EXPORT int spx_batch_use_graphs(SpxBatch *b, int on);
+
+spimage_EXPORT int spx_batch_iterate(SpxBatch *b, int iterations);
+spimage_EXPORT int spx_batch_iteration(const SpxBatch *b);
+
+/* per-reconstruction metrics, nrec values each (reference src/phasing.c:803-893) */
+spimage_EXPORT int spx_batch_efourier(SpxBatch *b, float *efourier, float *fcfo);
+spimage_EXPORT int spx_batch_ereal(SpxBatch *b, float *ereal);
+spimage_EXPORT int spx_batch_support_fraction(SpxBatch *b, float *fraction);
+
+/* results of reconstruction `rec` (rec < 0: all) */
+spimage_EXPORT int spx_batch_get_model(SpxBatch *b, int rec, float *model_complex);
+spimage_EXPORT int spx_batch_get_support(SpxBatch *b, int rec, int *support);
+
+/* device time of `iterations` iterations (no support updates), HIP events */
+spimage_EXPORT int spx_batch_time_iterations(SpxBatch *b, int iterations, float *ms);
+/* wait for all queued work of the batch */
+spimage_EXPORT int spx_batch_synchronize(SpxBatch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

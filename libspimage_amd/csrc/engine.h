@@ -89,6 +89,9 @@ class Engine {
   // Runs n iterations with betas[i] (ignored when per-reconstruction betas
   // are set); writes gp on the last one when want_gp.
   int iterate(const IterParams& p, const float* betas, int n, bool want_gp);
+  // n iterations (no gp) replayed from HIP graphs of one iteration, one per
+  // parity of the ping-pong iterates; recaptured when p or beta change
+  int iterate_graph(const IterParams& p, float beta, int n);
   void invalidate_R() { R_valid = false; Rf_valid = false; }
 
   // ---- metrics (reference src/phasing.c:803-893), nrec results each ----
@@ -143,6 +146,11 @@ class Engine {
   unsigned Rf_version = 0;
   float Rf_gamma1 = 0.f;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  IterParams gkey{};
+  float gbeta = 0.f;
+  bool gvalid = false;
+  void drop_graphs();
 };
 
 // one-shot transforms for the Image-level API (host buffers)

@@ -512,6 +512,7 @@ int Engine::alloc_all() {
 Engine::~Engine() {
   // teardown: errors here have nowhere to go
   if (s) (void)hipStreamSynchronize(s);
+  drop_graphs();
   void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min, amp_max,
                   absb, phased, flags, partials, red, hist, ctl, alg_rec, beta_rec};
   for (void* p : ptrs)
@@ -548,7 +549,10 @@ int Engine::download_flags(uint8_t* fl, int rec) {
   return 0;
 }
 
+// Buffers that appear on first use change the kernel arguments a captured
+// graph holds, so those setters drop the graphs.
 int Engine::upload_margins(const float* lo, const float* hi) {
+  drop_graphs();
   if (!amp_min) SPX_HIP(hipMalloc(&amp_min, sizeof(float) * NT));
   if (!amp_max) SPX_HIP(hipMalloc(&amp_max, sizeof(float) * NT));
   SPX_HIP(hipMemcpyAsync(amp_min, lo, sizeof(float) * NT, hipMemcpyHostToDevice, s));
@@ -558,6 +562,7 @@ int Engine::upload_margins(const float* lo, const float* hi) {
 }
 
 int Engine::upload_phased(const float2* ph, const uint8_t* fl) {
+  drop_graphs();
   if (!phased) SPX_HIP(hipMalloc(&phased, sizeof(float2) * NT));
   SPX_HIP(hipMemcpyAsync(phased, ph, sizeof(float2) * NT, hipMemcpyHostToDevice, s));
   SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)NT, hipMemcpyHostToDevice, s));
@@ -582,6 +587,7 @@ int Engine::download(const float2* dev, float2* host, int rec) {
 }
 
 int Engine::set_rec_params(const int* alg, const float* beta) {
+  drop_graphs();
   if (alg) {
     if (!alg_rec) SPX_HIP(hipMalloc(&alg_rec, sizeof(int) * nrec));
     SPX_HIP(hipMemcpyAsync(alg_rec, alg, sizeof(int) * nrec, hipMemcpyHostToDevice, s));
@@ -736,6 +742,59 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   } else {
     Rf_valid = false;
   }
+  return 0;
+}
+
+void Engine::drop_graphs() {
+  for (auto& x : gexec)
+    if (x) {
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+  gvalid = false;
+}
+
+int Engine::iterate_graph(const IterParams& p, float beta, int n) {
+  if (n <= 0) return 0;
+  if (int r = ensure_R()) return r;
+  if (p.alg == kDiffMap)
+    if (int r = ensure_Rf(p.gamma1)) return r;
+  const bool same = gvalid && gkey.alg == p.alg && gkey.constraints == p.constraints &&
+                    gkey.objective == p.objective && gkey.gamma1 == p.gamma1 && gkey.gamma2 == p.gamma2 &&
+                    gbeta == beta;
+  if (!same) {
+    drop_graphs();
+    const int cur0 = cur;
+    for (int parity = 0; parity < 2; ++parity) {
+      cur = parity;
+      hipGraph_t graph = nullptr;
+      SPX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int rc = iterate_impl(p, &beta, 1, false, nullptr);
+      const hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        cur = cur0;
+        set_error("graph capture failed");
+        return rc ? rc : -20;
+      }
+      const hipError_t ei = hipGraphInstantiate(&gexec[parity], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        cur = cur0;
+        set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+        return -20;
+      }
+    }
+    cur = cur0;
+    gkey = p;
+    gbeta = beta;
+    gvalid = true;
+  }
+  for (int i = 0; i < n; ++i) {
+    SPX_HIP(hipGraphLaunch(gexec[cur], s));
+    cur ^= 1;
+  }
+  if (p.alg != kDiffMap) Rf_valid = false;
   return 0;
 }
 

@@ -269,15 +269,20 @@ __device__ __forceinline__ void dft(float2* x) {
 // the lines (index = pos*G + g) so that lanes of one wavefront, which walk g
 // fastest, touch consecutive banks.
 // ---------------------------------------------------------------------------
+// One padding slot after every 16 positions: the stage-0 writes (stride =
+// radix, up to 16) and the early middle-stage writes then spread over the
+// banks instead of piling onto one.
 template <int L, int G, bool INTERLEAVED>
 struct LdsView {
-  static constexpr int LP = L + ((L >= 32) ? 1 : 0);  // odd pitch breaks power-of-two bank strides
-  static constexpr int SIZE = INTERLEAVED ? L * G : LP * G;
+  static constexpr int slot(int pos) { return pos + (pos >> 4); }
+  static constexpr int LS = L + (L >> 4);            // padded line length
+  static constexpr int LP = LS + ((LS % 2) ? 0 : 1);  // odd pitch between lines
+  static constexpr int SIZE = INTERLEAVED ? LS * G : LP * G;
   float2* base;
   int g;
   __device__ __forceinline__ float2& at(int pos) const {
-    if constexpr (INTERLEAVED) return base[pos * G + g];
-    else return base[g * LP + pos];
+    if constexpr (INTERLEAVED) return base[slot(pos) * G + g];
+    else return base[g * LP + slot(pos)];
   }
 };
 

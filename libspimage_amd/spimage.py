@@ -246,6 +246,143 @@ _bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)
 _bind("spx_phaser_last_threshold", C.c_int, PPhaser, C.POINTER(C.c_float))
 
 
+# batched phasing (include/sp_batch.h)
+PBatch = C.c_void_p
+PF = C.POINTER(C.c_float)
+PI = C.POINTER(C.c_int)
+_bind("spx_batch_create", PBatch, C.c_int, C.c_int, C.c_int, C.c_int)
+_bind("spx_batch_free", None, PBatch)
+_bind("spx_batch_size", C.c_int, PBatch)
+_bind("spx_batch_set_amplitudes", C.c_int, PBatch, C.c_int, PF, PI)
+_bind("spx_batch_set_support", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_set_model", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_init_model", C.c_int, PBatch, C.c_int, C.c_ulonglong)
+_bind("spx_batch_set_algorithm", C.c_int, PBatch, PAlg)
+_bind("spx_batch_set_rec_algorithms", C.c_int, PBatch, PI, PF)
+_bind("spx_batch_set_support_schedule", C.c_int, PBatch, PSupArr)
+_bind("spx_batch_use_graphs", C.c_int, PBatch, C.c_int)
+_bind("spx_batch_iterate", C.c_int, PBatch, C.c_int)
+_bind("spx_batch_iteration", C.c_int, PBatch)
+_bind("spx_batch_efourier", C.c_int, PBatch, PF, PF)
+_bind("spx_batch_ereal", C.c_int, PBatch, PF)
+_bind("spx_batch_support_fraction", C.c_int, PBatch, PF)
+_bind("spx_batch_get_model", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_get_support", C.c_int, PBatch, C.c_int, PI)
+_bind("spx_batch_time_iterations", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_synchronize", C.c_int, PBatch)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(PF)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(PI)
+
+
+class Batch:
+    """nrec same-shape reconstructions advanced together on the GPU.
+
+    Arrays are numpy, shaped like one image ((y, x) or (z, y, x)) for a
+    single reconstruction or with a leading reconstruction axis for all."""
+
+    def __init__(self, shape, nrec):
+        self.shape = tuple(int(s) for s in shape)
+        z, y, x = (1,) + self.shape if len(self.shape) == 2 else self.shape
+        self.nrec = nrec
+        self.h = spx_batch_create(x, y, z, nrec)
+        if not self.h:
+            raise RuntimeError(f"spx_batch_create failed: {last_error()}")
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            spx_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+    def set_data(self, rec, amplitudes, mask, support):
+        a = np.ascontiguousarray(amplitudes, np.float32)
+        m = np.ascontiguousarray(mask, np.int32)
+        s = np.ascontiguousarray(support, np.float32)
+        self._check(spx_batch_set_amplitudes(self.h, rec, _fptr(a), _iptr(m)), "set_amplitudes")
+        self._check(spx_batch_set_support(self.h, rec, _fptr(s)), "set_support")
+
+    def set_model(self, rec, model):
+        g = np.ascontiguousarray(model, np.complex64)
+        self._check(spx_batch_set_model(self.h, rec, g.ctypes.data_as(PF)), "set_model")
+
+    def init_model(self, flags, seed):
+        self._check(spx_batch_init_model(self.h, flags, seed), "init_model")
+
+    def set_algorithm(self, alg):
+        self._keep.append(alg)
+        self._check(spx_batch_set_algorithm(self.h, alg), "set_algorithm")
+
+    def set_rec_algorithms(self, algs=None, betas=None):
+        a = None if algs is None else np.ascontiguousarray(algs, np.int32)
+        b = None if betas is None else np.ascontiguousarray(betas, np.float32)
+        self._keep += [a, b]
+        self._check(spx_batch_set_rec_algorithms(self.h, None if a is None else _iptr(a),
+                                                 None if b is None else _fptr(b)), "set_rec_algorithms")
+
+    def set_support_schedule(self, sup):
+        self._keep.append(sup)
+        self._check(spx_batch_set_support_schedule(self.h, sup), "set_support_schedule")
+
+    def use_graphs(self, on=True):
+        self._check(spx_batch_use_graphs(self.h, 1 if on else 0), "use_graphs")
+
+    def iterate(self, n):
+        self._check(spx_batch_iterate(self.h, n), "iterate")
+
+    @property
+    def iteration(self):
+        return spx_batch_iteration(self.h)
+
+    def efourier(self):
+        e = np.empty(self.nrec, np.float32)
+        f = np.empty(self.nrec, np.float32)
+        self._check(spx_batch_efourier(self.h, _fptr(e), _fptr(f)), "efourier")
+        return e, f
+
+    def ereal(self):
+        e = np.empty(self.nrec, np.float32)
+        self._check(spx_batch_ereal(self.h, _fptr(e)), "ereal")
+        return e
+
+    def support_fraction(self):
+        e = np.empty(self.nrec, np.float32)
+        self._check(spx_batch_support_fraction(self.h, _fptr(e)), "support_fraction")
+        return e
+
+    def model(self, rec=-1):
+        n = self.nrec if rec < 0 else 1
+        out = np.empty((n,) + self.shape, np.complex64)
+        self._check(spx_batch_get_model(self.h, rec, out.ctypes.data_as(PF)), "get_model")
+        return out if rec < 0 else out[0]
+
+    def support(self, rec=-1):
+        n = self.nrec if rec < 0 else 1
+        out = np.empty((n,) + self.shape, np.int32)
+        self._check(spx_batch_get_support(self.h, rec, _iptr(out)), "get_support")
+        return out if rec < 0 else out[0]
+
+    def time_iterations(self, n):
+        ms = C.c_float()
+        self._check(spx_batch_time_iterations(self.h, n, C.byref(ms)), "time_iterations")
+        return ms.value
+
+    def synchronize(self):
+        self._check(spx_batch_synchronize(self.h), "synchronize")
+
+
 # ---------------------------------------------------------------------------
 # numpy helpers (the reference typemaps' orientation)
 # ---------------------------------------------------------------------------

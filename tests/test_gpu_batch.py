@@ -1,0 +1,126 @@
+"""The batched form (include/sp_batch.h) against single phasers.
+
+A batch runs the same per-line arithmetic as one phaser, so every
+reconstruction of a batch must equal the same reconstruction run alone,
+bit for bit (models, supports) and to float rounding for the reductions;
+graph replay must equal eager launches."""
+import numpy as np
+import pytest
+
+from _helpers import GpuPhaser
+
+pytestmark = pytest.mark.gpu
+
+
+def patterns(shape, n, seed=0):
+    from libspimage_amd import workloads
+    out = []
+    for r in range(n):
+        p = workloads.make_pattern(shape, n_gauss=6, radius=tuple(s / 6 for s in shape), photons=1e6, beamstop=2,
+                                   seed=seed + r)
+        out.append(p)
+    return out
+
+
+def run_single(sp, p, alg, n, beta=0.9, sup_array=None):
+    with GpuPhaser(sp, alg, p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start, beta=beta,
+                   sup_array=sup_array) as g:
+        assert g.iterate(n) == 0
+        return g.get("model"), g.get("support").real.astype(np.int32), g.efourier()[0]
+
+
+def make_batch(sp, pats, alg_ptr, sup=None, graphs=False):
+    b = sp.Batch(pats[0].amplitudes.shape, len(pats))
+    for r, p in enumerate(pats):
+        b.set_data(r, p.amplitudes, p.mask, p.support)
+        b.set_model(r, p.start)
+    b.set_algorithm(alg_ptr)
+    if sup is not None:
+        b.set_support_schedule(sup)
+    b.use_graphs(graphs)
+    return b
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (96, 80), (16, 16, 16)])
+def test_batch_equals_single_phasers(sp, shape):
+    pats = patterns(shape, 3, seed=40)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.iterate(12)
+    got = b.model()
+    ef, _ = b.efourier()
+    for r, p in enumerate(pats):
+        m, s, e = run_single(sp, p, "hio", 12)
+        assert np.array_equal(got[r], m), r
+        assert abs(ef[r] - e) <= 1e-6 * abs(e)
+    b.close()
+
+
+def test_batch_shrinkwrap_equals_single(sp):
+    pats = patterns((128, 128), 4, seed=50)
+    blur, area = [(0, 3.0), (30, 1.0)], [(0, 0.1)]
+    sup = sp.sp_support_array_init(sp.sp_support_area_alloc(sp.smap(blur), sp.smap(area)), 7)
+    raar = sp.sp_phasing_raar_alloc(sp.smap([(0, 0.87)]), 0)
+    b = make_batch(sp, pats, raar, sup)
+    b.iterate(30)
+    assert b.iteration == 30
+    got, gsup = b.model(), b.support()
+    frac = b.support_fraction()
+    for r, p in enumerate(pats):
+        sup1 = sp.sp_support_array_init(sp.sp_support_area_alloc(sp.smap(blur), sp.smap(area)), 7)
+        m, s, _ = run_single(sp, p, "raar", 30, beta=0.87, sup_array=sup1)
+        assert np.array_equal(gsup[r], s), r
+        assert np.array_equal(got[r], m), r
+        assert abs(frac[r] - s.mean()) < 1e-6
+    b.close()
+
+
+def test_per_reconstruction_algorithms(sp):
+    pats = patterns((64, 64), 3, seed=60)
+    algs = np.array([4, 1, 2], np.int32)  # ER, HIO, RAAR
+    betas = np.array([0.0, 0.8, 0.6], np.float32)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.set_rec_algorithms(algs, betas)
+    b.iterate(8)
+    got = b.model()
+    for r, (name, beta) in enumerate([("er", 0.0), ("hio", 0.8), ("raar", 0.6)]):
+        m, _, _ = run_single(sp, pats[r], name, 8, beta=beta)
+        assert np.array_equal(got[r], m), name
+    b.close()
+
+
+def test_graph_replay_equals_eager(sp):
+    """Graph replays interleaved with eager work and synchronisations give the
+    same iterates as eager launches, on every chunk."""
+    pats = patterns((128, 96), 3, seed=70)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b_eager = make_batch(sp, pats, hio, graphs=False)
+    b_graph = make_batch(sp, pats, hio, graphs=True)
+    for chunk in (5, 9, 4, 11):
+        b_eager.iterate(chunk)
+        b_graph.iterate(chunk)
+        b_graph.synchronize()
+        e1, _ = b_graph.efourier()  # eager work between replays
+        assert np.array_equal(b_graph.model(), b_eager.model()), chunk
+        e0, _ = b_eager.efourier()
+        assert np.array_equal(e0, e1)
+    b_eager.close()
+    b_graph.close()
+
+
+def test_batch_random_start_streams_differ(sp):
+    pats = patterns((64, 64), 2, seed=80)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.init_model(sp.SpModelRandomPhases, 99)
+    m = b.model()
+    for r, p in enumerate(pats):
+        F = np.fft.fftn(m[r].astype(np.complex128))
+        assert np.allclose(np.abs(F), p.amplitudes, atol=2e-5 * p.amplitudes.max())
+    # same amplitudes in both slots still give different phases
+    b.set_data(1, pats[0].amplitudes, pats[0].mask, pats[0].support)
+    b.init_model(sp.SpModelRandomPhases, 99)
+    m = b.model()
+    assert not np.allclose(m[0], m[1])
+    b.close()
