@@ -116,6 +116,7 @@ def gpu_bench(args):
         "achieved": it_achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": it_achieved / PEAK_HBM,
         "algorithmic_bytes": BYTES_HIO * N,
         "passes_us": {k: v["seconds"] * 1e6 for k, v in passes.items()},
+        "fft_elements_per_thread": dict(zip(("row", "column"), sp.fft_variant(1024))),
     }
 
     # the 1024^2 reconstruction: RAAR beta 0.87, 300 iterations, area

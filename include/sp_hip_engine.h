@@ -27,6 +27,9 @@ spimage_EXPORT int spx_device_available(void);
 spimage_EXPORT const char *spx_last_error(void);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
+/* Elements per thread (4, 8 or 16) of the transform variant the tuner chose
+ * for lines of length L in row passes and in column passes. */
+spimage_EXPORT int spx_fft_variant(int L, int *row_elements, int *col_elements);
 
 /* Unnormalised c2c transform of an (nz, ny, nx) x-fastest complex64 array in
  * device memory; direction -1 forward, +1 inverse; d_in == d_out allowed.

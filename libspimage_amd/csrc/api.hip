@@ -968,6 +968,17 @@ int spx_device_available(void) { return spx::device_available() ? 1 : 0; }
 const char* spx_last_error(void) { return spx::last_error(); }
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
+int spx_fft_variant(int L, int* row_e, int* col_e) {
+  const spx::LengthEntry* e = spx::find_length(L);
+  if (!e) {
+    spx::set_error("length %d is not supported", L);
+    return -11;
+  }
+  if (row_e) *row_e = e->var[e->row_choice].E;
+  if (col_e) *col_e = e->var[e->col_choice].E;
+  return 0;
+}
+
 int spx_fft_device(void* d_in, void* d_out, int nx, int ny, int nz, int direction) {
   if (direction != -1 && direction != 1) {
     spx::set_error("direction must be -1 or +1");

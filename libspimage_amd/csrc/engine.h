@@ -34,8 +34,11 @@ const char* last_error();
   } while (0)
 
 bool device_available();
-// device twiddle table of a supported length (built once per process)
-const float2* twiddles(int L);
+// Chooses, once per process and length, the plan variant (elements per
+// thread) of row passes and of column passes by timing every compiled
+// variant on the given geometry.  SPX_FFT_E=4|8|16 forces a variant and
+// SPX_NO_TUNE=1 keeps the default (8).
+void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo);
 
 struct IterParams {
   int alg;            // SpPhasingAlgorithmType
@@ -48,7 +51,6 @@ struct IterParams {
 struct Shape {
   int nx, ny, nz, nrec;
   const LengthEntry *ex, *ey, *ez;
-  const float2 *twx, *twy, *twz;
   LineGeom gy, gz;
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);

@@ -36,17 +36,18 @@ bool device_available() {
 }
 
 // ---------------------------------------------------------------------------
-// twiddle tables: for stage s >= 1 of a length-L plan, entry
+// twiddle tables: for stage s >= 1 of plan (L, E), entry
 // twoff[s] + j*tw_stride(R) + (k-1) holds exp(-2 pi i j k / (Ns R)),
 // evaluated in double and rounded to float (rows padded to an even length)
 // ---------------------------------------------------------------------------
-const float2* twiddles(int L) {
+const float2* twiddles(int L, int E) {
   static std::mutex mu;
   static std::map<int, float2*> tables;
   std::lock_guard<std::mutex> lock(mu);
-  auto it = tables.find(L);
+  const int key = L * 64 + E;
+  auto it = tables.find(key);
   if (it != tables.end()) return it->second;
-  const FftPlan p = make_plan(L);
+  const FftPlan p = make_plan(L, E);
   std::vector<float2> h(p.twtotal > 0 ? p.twtotal : 2, make_float2(0.f, 0.f));
   for (int s = 1; s < p.n; ++s) {
     const int R = p.r[s], Ns = p.ns[s];
@@ -59,8 +60,113 @@ const float2* twiddles(int L) {
   float2* d = nullptr;
   if (hipMalloc(&d, h.size() * sizeof(float2)) != hipSuccess) return nullptr;
   if (hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  tables[L] = d;
+  tables[key] = d;
   return d;
+}
+
+// ---------------------------------------------------------------------------
+// plan-variant tuner
+// ---------------------------------------------------------------------------
+static int variant_index(const LengthEntry* e, int E) {
+  for (int i = 0; i < e->nvar; ++i)
+    if (e->var[i].E == E) return i;
+  return -1;
+}
+
+// Milliseconds per launch of `launch` (median-free: best of a few batches).
+template <class F>
+static float time_launches(hipStream_t s, F&& launch) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return 1e30f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return 1e30f;
+  }
+  launch();
+  launch();
+  float best = 1e30f;
+  for (int rep = 0; rep < 3; ++rep) {
+    (void)hipEventRecord(a, s);
+    for (int i = 0; i < 8; ++i) launch();
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess && ms / 8 < best) best = ms / 8;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (hipGetLastError() == hipSuccess) ? best : 1e30f;
+}
+
+void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
+  static std::mutex mu;
+  static std::map<LengthEntry*, int> done;  // bit 1 row tuned, bit 2 column tuned
+  std::lock_guard<std::mutex> lock(mu);
+  const int bit = col_geo ? 2 : 1;
+  if (done[e] & bit) return;
+  done[e] |= bit;
+  int choice = variant_index(e, 8);
+  if (choice < 0) choice = 0;
+  const char* force = getenv("SPX_FFT_E");
+  const char* notune = getenv("SPX_NO_TUNE");
+  if (force && variant_index(e, atoi(force)) >= 0) {
+    choice = variant_index(e, atoi(force));
+  } else if (e->nvar > 1 && !(notune && atoi(notune) == 1)) {
+    // time the hot kernels of the pass on zero-filled scratch of the real geometry
+    const long L = e->L;
+    long n;
+    if (col_geo) n = (long)(col_geo->nouter - 1) * col_geo->outer_stride + (L - 1) * col_geo->axis_stride + col_geo->ncontig;
+    else n = (long)lines * L;
+    hipStream_t s = nullptr;
+    float2 *b0 = nullptr, *b1 = nullptr, *b2 = nullptr;
+    float* amp = nullptr;
+    uint8_t* fl = nullptr;
+    bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&b0, sizeof(float2) * n) == hipSuccess && hipMalloc(&b1, sizeof(float2) * n) == hipSuccess &&
+              hipMalloc(&b2, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+              hipMalloc(&fl, (size_t)n) == hipSuccess;
+    ok = ok && hipMemsetAsync(b0, 0, sizeof(float2) * n, s) == hipSuccess &&
+         hipMemsetAsync(b1, 0, sizeof(float2) * n, s) == hipSuccess &&
+         hipMemsetAsync(amp, 0, sizeof(float) * n, s) == hipSuccess && hipMemsetAsync(fl, 0, (size_t)n, s) == hipSuccess;
+    if (ok) {
+      float best = 1e30f;
+      for (int i = 0; i < e->nvar; ++i) {
+        const Variant& v = e->var[i];
+        float ms;
+        if (col_geo) {
+          ColArgs c{};
+          c.in = b0;
+          c.W = b0;
+          c.geo = *col_geo;
+          c.ampx = amp;
+          ms = time_launches(s, [&] { v.col(kColProject, c, s); });
+        } else {
+          UpdateArgs u{};
+          u.W = b0;
+          u.gcur = b1;
+          u.gnew = b2;
+          u.flags = fl;
+          u.nlines = lines;
+          u.lines_per_rec = lines;
+          u.alg = kER;
+          u.scale = 1.f;
+          u.nsize = 1.f;
+          ms = time_launches(s, [&] { v.update(u, false, s); });
+        }
+        if (ms < best) {
+          best = ms;
+          choice = i;
+        }
+      }
+    }
+    (void)hipStreamSynchronize(s);
+    void* bufs[] = {b0, b1, b2, amp, fl};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  if (col_geo) e->col_choice = choice;
+  else e->row_choice = choice;
 }
 
 // ---------------------------------------------------------------------------
@@ -346,24 +452,23 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   }
   const int dims[3] = {nx, ny, nz};
   const LengthEntry** e[3] = {&sh->ex, &sh->ey, &sh->ez};
-  const float2** t[3] = {&sh->twx, &sh->twy, &sh->twz};
+  LengthEntry* found[3] = {nullptr, nullptr, nullptr};
   for (int a = 0; a < 3; ++a) {
     if (dims[a] == 1) continue;
-    *e[a] = find_length(dims[a]);
-    if (!*e[a]) {
+    found[a] = find_length(dims[a]);
+    if (!found[a]) {
       set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
       return -11;
     }
-    *t[a] = twiddles(dims[a]);
-    if (!*t[a]) {
-      set_error("could not build twiddle table for length %d", dims[a]);
-      return -20;
-    }
+    *e[a] = found[a];
   }
   // y lines: (rec, z) outer blocks of nx adjacent lines; z lines: rec outer
   // blocks of nx*ny adjacent lines
   sh->gy = LineGeom{nx, nz * nrec, (long)nx * ny, (long)nx};
   sh->gz = LineGeom{nx * ny, nrec, (long)nx * ny * nz, (long)nx * ny};
+  if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
+  if (found[1]) tune_length(found[1], 0, &sh->gy);
+  if (found[2]) tune_length(found[2], 0, &sh->gz);
   return 0;
 }
 
@@ -371,7 +476,6 @@ static RowArgs row_args(const Shape& sh, const float2* in, float2* out) {
   RowArgs r{};
   r.in = in;
   r.out = out;
-  r.tw = sh.twx;
   r.nlines = sh.ny * sh.nz * sh.nrec;
   r.lines_per_rec = sh.ny * sh.nz;
   r.L = sh.nx;
@@ -389,7 +493,6 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
   if (sh.ey) {
     c.in = src;
     c.W = out;
-    c.tw = sh.twy;
     c.geo = sh.gy;
     sh.ey->col(dir < 0 ? kColFwd : kColInv, c, s);
     src = out;
@@ -397,7 +500,6 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
   if (sh.ez) {
     c.in = src;
     c.W = out;
-    c.tw = sh.twz;
     c.geo = sh.gz;
     sh.ez->col(dir < 0 ? kColFwd : kColInv, c, s);
     src = out;
@@ -416,27 +518,23 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
   if (three) {
     a.in = in;
     a.W = out;
-    a.tw = sh.twy;
     a.geo = sh.gy;
     sh.ey->col(kColFwd, a, s);
     a = base;
     a.in = out;
     a.W = out;
-    a.tw = sh.twz;
     a.geo = sh.gz;
     sh.ez->col(mode, a, s);
     if (mode != kColEFourier) {
       a = base;
       a.in = out;
       a.W = out;
-      a.tw = sh.twy;
-      a.geo = sh.gy;
+        a.geo = sh.gy;
       sh.ey->col(kColInv, a, s);
     }
   } else {
     a.in = in;
     a.W = out;
-    a.tw = sh.ez ? sh.twz : sh.twy;
     a.geo = sh.ez ? sh.gz : sh.gy;
     (sh.ez ? sh.ez : sh.ey)->col(mode, a, s);
   }
@@ -711,7 +809,6 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.W = W;
   u.Wf = Wf;
   u.flags = flags;
-  u.tw = sh.twx;
   u.beta_rec = beta_rec;
   u.alg_rec = dm ? nullptr : alg_rec;
   u.nlines = ny * nz * nrec;
@@ -944,14 +1041,12 @@ int Engine::patterson_support(const float* amp_raw, float value) {
   if (sh.ey) {
     c.in = S1;
     c.W = S1;
-    c.tw = sh.twy;
     c.geo = sh.gy;
     sh.ey->col(kColInv, c, s);
   }
   if (sh.ez) {
     c.in = S1;
     c.W = S1;
-    c.tw = sh.twz;
     c.geo = sh.gz;
     sh.ez->col(kColInv, c, s);
   }

@@ -56,20 +56,23 @@ constexpr int gcd_c(int a, int b) {
 }
 constexpr int pow2_stages(int e) { return (e + 3) / 4; }
 
-// Appends the radices of 2^e, largest first (16, 8, 4, 2).
-constexpr void push_pow2(FftPlan& p, int e) {
-  while (e > 0) {
-    int take = 0;
-    if (e >= 8 || e == 4 || e == 7) take = 4;
-    else if (e >= 3) take = 3;
-    else take = e;
-    if (e == 5) take = 3;
+// Appends the radices of 2^m (each at most 2^kmax), as even as possible,
+// larger first: 2^6 with kmax 2 gives 4,4,4; 2^5 gives 4,4,2.
+constexpr void push_pow2(FftPlan& p, int m, int kmax) {
+  if (m <= 0) return;
+  const int k = (m + kmax - 1) / kmax;
+  int left = m;
+  for (int i = 0; i < k; ++i) {
+    const int take = (left + (k - i) - 1) / (k - i);
     p.r[p.n++] = 1 << take;
-    e -= take;
+    left -= take;
   }
 }
 
-constexpr FftPlan make_plan(int L) {
+// Plan of a length-L line whose threads hold E elements at the edges
+// (E = 4, 8 or 16; also the largest power-of-two radix used).  The edge
+// radix 2^ke appears first and last so the transform is palindromic.
+constexpr FftPlan make_plan(int L, int E) {
   FftPlan p{};
   p.L = L;
   p.n = 0;
@@ -81,31 +84,19 @@ constexpr FftPlan make_plan(int L) {
     p.ok = false;
     return p;
   }
-  const int a2 = ilog2c(rem);
-  if (L == 2 || L == 3 || L == 4 || L == 5 || L == 8 || L == 16) {
+  const int a2 = ilog2c(rem), kE = ilog2c(E);
+  if (L == 3 || L == 5 || (L == rem && L <= E)) {
     p.r[p.n++] = L;
   } else {
-    // palindromic edges 2^ke ... 2^ke with the fewest stages, larger edges
-    // on ties; a plain plan when that saves a stage even after the LDS
-    // round-trip it costs between chained transforms
-    int best_ke = 0, best = 1 << 20;
-    for (int ke = 4; ke >= 1; --ke) {
-      if (2 * ke > a2) continue;
-      const int st = 2 + pow2_stages(a2 - 2 * ke) + n3 + n5;
-      if (st < best) {
-        best = st;
-        best_ke = ke;
-      }
-    }
-    const int plain = pow2_stages(a2) + n3 + n5;
-    if (best_ke > 0 && best <= plain + 1) {
-      p.r[p.n++] = 1 << best_ke;
-      push_pow2(p, a2 - 2 * best_ke);
+    const int ke = (a2 / 2 < kE) ? a2 / 2 : kE;
+    if (ke >= 1) {
+      p.r[p.n++] = 1 << ke;
+      push_pow2(p, a2 - 2 * ke, kE);
       for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
       for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
-      p.r[p.n++] = 1 << best_ke;
+      p.r[p.n++] = 1 << ke;
     } else {
-      push_pow2(p, a2);
+      push_pow2(p, a2, kE);
       for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
       for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
     }
@@ -121,26 +112,29 @@ constexpr FftPlan make_plan(int L) {
   p.twtotal = off;
   p.maxr = mr;
   const int e0 = p.r[0], el = p.r[p.n - 1];
-  const int lcm = e0 / gcd_c(e0, el) * el;
-  const int m = (mr % e0 == 0 && mr % el == 0) ? mr : lcm;
-  p.T = L / m;
+  p.T = L / (e0 / gcd_c(e0, el) * el);
   if (ns != L) p.ok = false;
   return p;
 }
 
-template <int L>
+template <int L, int E>
 struct PlanOf {
-  static constexpr FftPlan P = make_plan(L);
+  static constexpr FftPlan P = make_plan(L, E);
   static_assert(P.ok, "transform length does not factor into radices 2,3,4,5,8,16");
   static constexpr int T = P.T;
   static constexpr int E0 = P.r[0];
   static constexpr int EL = P.r[P.n - 1];
-  static constexpr int NREG = L / T;     // elements each thread holds at the edges
+  static constexpr int NREG = L / T;       // elements each thread holds at the edges
   static constexpr int B0 = (L / E0) / T;  // stage-0 butterflies per thread
   static constexpr int BL = (L / EL) / T;  // last-stage butterflies per thread
   static constexpr bool PAL = (E0 == EL);  // output positions == input positions
+  static constexpr int TW = P.twtotal > 0 ? P.twtotal : 2;  // twiddle entries (even)
   static_assert(B0 * E0 == NREG && BL * EL == NREG, "edge stages must divide evenly over the threads");
 };
+
+// the edge registers of one thread
+template <int L, int E>
+using RegLine = float2[PlanOf<L, E>::NREG];
 
 // ---------------------------------------------------------------------------
 // Complex helpers on float2 (x = re, y = im).
@@ -301,9 +295,9 @@ __device__ __forceinline__ void load_twiddles(const float2* __restrict__ row, fl
 }
 
 // Middle stage S: LDS `in` -> LDS `out` (no barrier inside).
-template <int L, int S, int DIR, class View>
+template <int L, int E, int S, int DIR, class View>
 __device__ __forceinline__ void mid_stage(const View& in, const View& out, int t, const float2* __restrict__ tw) {
-  constexpr FftPlan P = PlanOf<L>::P;
+  constexpr FftPlan P = PlanOf<L, E>::P;
   constexpr int R = P.r[S], NS = P.ns[S], T = P.T;
   constexpr int NB = L / R;
   constexpr int BPT = (NB + T - 1) / T;
@@ -327,15 +321,15 @@ __device__ __forceinline__ void mid_stage(const View& in, const View& out, int t
   }
 }
 
-template <int L, int S, int DIR, class View>
+template <int L, int E, int S, int DIR, class View>
 __device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, const float2* __restrict__ tw) {
-  constexpr FftPlan P = PlanOf<L>::P;
+  constexpr FftPlan P = PlanOf<L, E>::P;
   if constexpr (S < P.n - 1) {
     // stage S reads the buffer stage S-1 wrote: a for odd S, b for even S
-    if constexpr (S % 2 == 1) mid_stage<L, S, DIR>(a, b, t, tw);
-    else mid_stage<L, S, DIR>(b, a, t, tw);
+    if constexpr (S % 2 == 1) mid_stage<L, E, S, DIR>(a, b, t, tw);
+    else mid_stage<L, E, S, DIR>(b, a, t, tw);
     __syncthreads();
-    mid_stages<L, S + 1, DIR>(a, b, t, tw);
+    mid_stages<L, E, S + 1, DIR>(a, b, t, tw);
   }
 }
 
@@ -344,15 +338,15 @@ __device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, 
 // at out_pos(t, i, r) = (t + i T) + r L/EL.  Uses LDS buffers a and b; ends
 // without a barrier (the last stage only reads LDS), so a caller that writes
 // LDS next must synchronise first.
-template <int L, int DIR, class View>
-__device__ __forceinline__ void fft_line(float2 (&v)[PlanOf<L>::NREG], const View& a, const View& b, int t,
+template <int L, int E, int DIR, class View>
+__device__ __forceinline__ void fft_line(RegLine<L, E>& v, const View& a, const View& b, int t,
                                          const float2* __restrict__ tw) {
-  constexpr FftPlan P = PlanOf<L>::P;
+  constexpr FftPlan P = PlanOf<L, E>::P;
   constexpr int T = P.T;
   if constexpr (P.n == 1) {
     dft<L, DIR>(v);
   } else {
-    constexpr int E0 = P.r[0], B0 = PlanOf<L>::B0;
+    constexpr int E0 = P.r[0], B0 = PlanOf<L, E>::B0;
     // stage 0: no twiddles (Ns = 1); outputs land at b*E0 + r
 #pragma unroll
     for (int i = 0; i < B0; ++i) {
@@ -362,10 +356,10 @@ __device__ __forceinline__ void fft_line(float2 (&v)[PlanOf<L>::NREG], const Vie
       for (int r = 0; r < E0; ++r) a.at(bb * E0 + r) = v[i * E0 + r];
     }
     __syncthreads();
-    mid_stages<L, 1, DIR>(a, b, t, tw);
+    mid_stages<L, E, 1, DIR>(a, b, t, tw);
     // last stage reads the buffer written by stage n-2
     constexpr int SL = P.n - 1;
-    constexpr int EL = P.r[SL], NS = P.ns[SL], BL = PlanOf<L>::BL;
+    constexpr int EL = P.r[SL], NS = P.ns[SL], BL = PlanOf<L, E>::BL;
     const View& src = (SL % 2 == 1) ? a : b;
 #pragma unroll
     for (int i = 0; i < BL; ++i) {
@@ -384,11 +378,11 @@ __device__ __forceinline__ void fft_line(float2 (&v)[PlanOf<L>::NREG], const Vie
 
 // Moves the outputs of one transform to the input positions of the next
 // when the plan's edge radices differ (a no-op for palindromic plans).
-template <int L, class View>
-__device__ __forceinline__ void out_to_in(float2 (&v)[PlanOf<L>::NREG], const View& a, int t) {
-  if constexpr (!PlanOf<L>::PAL) {
-    constexpr int T = PlanOf<L>::T, E0 = PlanOf<L>::E0, EL = PlanOf<L>::EL;
-    constexpr int B0 = PlanOf<L>::B0, BL = PlanOf<L>::BL;
+template <int L, int E, class View>
+__device__ __forceinline__ void out_to_in(RegLine<L, E>& v, const View& a, int t) {
+  if constexpr (!PlanOf<L, E>::PAL) {
+    constexpr int T = PlanOf<L, E>::T, E0 = PlanOf<L, E>::E0, EL = PlanOf<L, E>::EL;
+    constexpr int B0 = PlanOf<L, E>::B0, BL = PlanOf<L, E>::BL;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < BL; ++i)
@@ -403,14 +397,14 @@ __device__ __forceinline__ void out_to_in(float2 (&v)[PlanOf<L>::NREG], const Vi
 }
 
 // positions of the edge registers
-template <int L>
+template <int L, int E>
 __device__ __forceinline__ int in_pos(int t, int q) {
-  constexpr int E0 = PlanOf<L>::E0, T = PlanOf<L>::T;
+  constexpr int E0 = PlanOf<L, E>::E0, T = PlanOf<L, E>::T;
   return (t + (q / E0) * T) + (q % E0) * (L / E0);
 }
-template <int L>
+template <int L, int E>
 __device__ __forceinline__ int out_pos(int t, int q) {
-  constexpr int EL = PlanOf<L>::EL, T = PlanOf<L>::T;
+  constexpr int EL = PlanOf<L, E>::EL, T = PlanOf<L, E>::T;
   return (t + (q / EL) * T) + (q % EL) * (L / EL);
 }
 

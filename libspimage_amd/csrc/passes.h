@@ -204,23 +204,35 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV
   }
 }
 
+
+// Copies the length's twiddle table into LDS (issued before the data loads
+// complete; the first barrier inside fft_line orders it before first use).
+template <int L, int E, int NT>
+__device__ __forceinline__ void stage_twiddles(float4* lds, const float2* __restrict__ tw) {
+  constexpr int N4 = PlanOf<L, E>::TW / 2;
+  const float4* g = reinterpret_cast<const float4*>(tw);
+  for (int i = threadIdx.x; i < N4; i += NT) lds[i] = g[i];
+}
+
 // ---------------------------------------------------------------------------
 // Row kernels: M lines along x per work-group, T threads per line.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, int E>
 struct RowShape {
-  static constexpr int T = PlanOf<L>::T;
-  static constexpr int M = (T >= 64) ? 1 : (64 / T);
+  static constexpr int T = PlanOf<L, E>::T;
+  static constexpr int M = (T >= 256) ? 1 : (256 / T);
   static constexpr int NT = T * M;
 };
 
-template <int L, int MODE>
-__global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
-  constexpr int T = RowShape<L>::T, M = RowShape<L>::M, NT = RowShape<L>::NT;
-  constexpr int NR = PlanOf<L>::NREG;
+template <int L, int E, int MODE>
+__global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
+  using PL = PlanOf<L, E>;
+  constexpr int T = RowShape<L, E>::T, M = RowShape<L, E>::M, NT = RowShape<L, E>::NT;
+  constexpr int NR = PL::NREG;
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
+  __shared__ float4 tws[PL::TW / 2];
   __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
@@ -230,22 +242,24 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
   const View va{shA, g}, vb{shB, g};
   const long base = (long)line * L;
 
-  if constexpr (MODE == kRowBlurOut) {
-    if (a.hist)
-      for (int i = tid; i < 2048; i += NT) hsh[i] = 0u;
-  }
   float2 v[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     float2 x = make_float2(0.f, 0.f);
     if (act) {
-      x = a.in[base + in_pos<L>(t, q)];
+      x = a.in[base + in_pos<L, E>(t, q)];
       if constexpr (MODE == kRowAbsFwd) x = make_float2(cabsf_ref(x), 0.f);
     }
     v[q] = x;
   }
-  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, +1>(v, va, vb, t, a.tw);
-  else fft_line<L, -1>(v, va, vb, t, a.tw);
+  stage_twiddles<L, E, NT>(tws, a.tw);
+  if constexpr (MODE == kRowBlurOut) {
+    if (a.hist)
+      for (int i = tid; i < 2048; i += NT) hsh[i] = 0u;
+  }
+  const float2* twl = reinterpret_cast<const float2*>(tws);
+  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, E, +1>(v, va, vb, t, twl);
+  else fft_line<L, E, -1>(v, va, vb, t, twl);
 
   if constexpr (MODE == kRowBlurOut) {
     // the work-group's lines belong to one reconstruction unless lines of
@@ -261,7 +275,7 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
     for (int q = 0; q < NR; ++q) {
       if (act) {
         const float m = cabsf_ref(v[q]);
-        a.absb[base + out_pos<L>(t, q)] = m;
+        a.absb[base + out_pos<L, E>(t, q)] = m;
         mx = fmaxf(mx, m);
         if (a.hist) {
           if (one_rec) atomicAdd(&hsh[__float_as_uint(m) >> 21], 1u);
@@ -287,7 +301,7 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
   } else {
 #pragma unroll
     for (int q = 0; q < NR; ++q)
-      if (act) a.out[base + out_pos<L>(t, q)] = v[q];
+      if (act) a.out[base + out_pos<L, E>(t, q)] = v[q];
   }
 }
 
@@ -295,14 +309,17 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row(RowArgs a) {
 // column-pass output, 1/N scale, algorithm update, support constraints, store
 // of the new iterate, then forward x-transform of the new iterate (and, for
 // the difference map, of its f1) for the next iteration's column pass.  The
-// update runs on registers at the inverse transform's output positions.
-template <int L, bool DM>
-__global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
-  constexpr int T = RowShape<L>::T, M = RowShape<L>::M;
-  constexpr int NR = PlanOf<L>::NREG;
+// update runs on registers at the inverse transform's output positions; its
+// other operands (g_k, flags) are loaded before the inverse transform.
+template <int L, int E, bool DM>
+__global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs a) {
+  using PL = PlanOf<L, E>;
+  constexpr int T = RowShape<L, E>::T, M = RowShape<L, E>::M, NT = RowShape<L, E>::NT;
+  constexpr int NR = PL::NREG;
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
+  __shared__ float4 tws[PL::TW / 2];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -311,16 +328,25 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
   const View va{shA, g}, vb{shB, g};
   const long base = (long)line * L;
 
-  float2 v[NR], vf[DM ? NR : 1];
+  float2 v[NR], vf[DM ? NR : 1], g0v[NR];
+  uint8_t fl[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    v[q] = act ? a.W[base + in_pos<L>(t, q)] : make_float2(0.f, 0.f);
-    if constexpr (DM) vf[q] = act ? a.Wf[base + in_pos<L>(t, q)] : make_float2(0.f, 0.f);
+    v[q] = act ? a.W[base + in_pos<L, E>(t, q)] : make_float2(0.f, 0.f);
+    if constexpr (DM) vf[q] = act ? a.Wf[base + in_pos<L, E>(t, q)] : make_float2(0.f, 0.f);
   }
-  fft_line<L, +1>(v, va, vb, t, a.tw);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const long idx = base + out_pos<L, E>(t, q);
+    g0v[q] = act ? a.gcur[idx] : make_float2(0.f, 0.f);
+    fl[q] = act ? a.flags[idx] : (uint8_t)0;
+  }
+  stage_twiddles<L, E, NT>(tws, a.tw);
+  const float2* twl = reinterpret_cast<const float2*>(tws);
+  fft_line<L, E, +1>(v, va, vb, t, twl);
   if constexpr (DM) {
     __syncthreads();
-    fft_line<L, +1>(vf, va, vb, t, a.tw);
+    fft_line<L, E, +1>(vf, va, vb, t, twl);
   }
 
   const int rec = act ? line / a.lines_per_rec : 0;
@@ -329,10 +355,10 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (!act) continue;
-    const long idx = base + out_pos<L>(t, q);
+    const long idx = base + out_pos<L, E>(t, q);
     const float2 P = v[q];
-    const float2 g0 = a.gcur[idx];
-    const bool in = a.flags[idx] & kInsideSupport;
+    const float2 g0 = g0v[q];
+    const bool in = fl[q] & kInsideSupport;
     float2 g1;
     if constexpr (!DM) {
       const float2 gp = make_float2(P.x * a.scale, P.y * a.scale);
@@ -366,42 +392,49 @@ __global__ __launch_bounds__(RowShape<L>::NT) void k_row_update(UpdateArgs a) {
     // f1 of the next iteration (src/phasing.c:1260-1274)
     if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
   }
-  out_to_in<L>(v, va, t);
-  if constexpr (DM) out_to_in<L>(vf, va, t);
+  out_to_in<L, E>(v, va, t);
+  if constexpr (DM) out_to_in<L, E>(vf, va, t);
   __syncthreads();
-  fft_line<L, -1>(v, va, vb, t, a.tw);
+  fft_line<L, E, -1>(v, va, vb, t, twl);
   if constexpr (DM) {
     __syncthreads();
-    fft_line<L, -1>(vf, va, vb, t, a.tw);
+    fft_line<L, E, -1>(vf, va, vb, t, twl);
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (!act) continue;
-    a.W[base + out_pos<L>(t, q)] = v[q];
-    if constexpr (DM) a.Wf[base + out_pos<L>(t, q)] = vf[q];
+    a.W[base + out_pos<L, E>(t, q)] = v[q];
+    if constexpr (DM) a.Wf[base + out_pos<L, E>(t, q)] = vf[q];
   }
 }
 
 // ---------------------------------------------------------------------------
 // Column kernels: C adjacent lines (contiguous in memory) per work-group.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, int E>
 struct ColShape {
-  static constexpr int T = PlanOf<L>::T;
-  // two LDS buffers of L*C complex values within 64 KiB
-  static constexpr int Cmem = (4096 / L) >= 1 ? (4096 / L) : 1;
-  static constexpr int C1 = Cmem < 16 ? Cmem : 16;
-  static constexpr int C = (T * C1 >= 64) ? C1 : ((64 / T) < Cmem ? (64 / T) : Cmem);
+  static constexpr int T = PlanOf<L, E>::T;
+  // C >= 4 adjacent lines (32-byte segments) and about 256 threads, within
+  // 1024 threads and about 96 KiB of LDS
+  static constexpr int LS = L + (L >> 4);
+  static constexpr int Cthr = (1024 / T) >= 1 ? (1024 / T) : 1;
+  static constexpr int Cmem = (98304 / (2 * LS * 8)) >= 1 ? (98304 / (2 * LS * 8)) : 1;
+  static constexpr int Cw = (256 / T) > 4 ? (256 / T) : 4;
+  static constexpr int C1 = Cw < 64 ? Cw : 64;
+  static constexpr int C2 = C1 < Cthr ? C1 : Cthr;
+  static constexpr int C = C2 < Cmem ? C2 : Cmem;
   static constexpr int NT = T * C;
 };
 
-template <int L, int MODE>
-__global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
-  constexpr int T = ColShape<L>::T, C = ColShape<L>::C, NT = ColShape<L>::NT;
-  constexpr int NR = PlanOf<L>::NREG;
+template <int L, int E, int MODE>
+__global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
+  using PL = PlanOf<L, E>;
+  constexpr int T = ColShape<L, E>::T, C = ColShape<L, E>::C, NT = ColShape<L, E>::NT;
+  constexpr int NR = PL::NREG;
   using View = LdsView<L, C, true>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
+  __shared__ float4 tws[PL::TW / 2];
   __shared__ double red[(MODE == kColEFourier) ? 4 * 32 : 1];
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
@@ -416,27 +449,37 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
 
   float2 v[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) v[q] = act ? a.in[base + in_pos<L>(t, q) * st] : make_float2(0.f, 0.f);
-  if constexpr (MODE == kColInv) fft_line<L, +1>(v, va, vb, t, a.tw);
-  else fft_line<L, -1>(v, va, vb, t, a.tw);
+  for (int q = 0; q < NR; ++q) v[q] = act ? a.in[base + in_pos<L, E>(t, q) * st] : make_float2(0.f, 0.f);
+  // amplitudes of the projection, fetched with the data
+  float ampv[(MODE == kColProject || MODE == kColEFourier) ? NR : 1];
+  if constexpr (MODE == kColProject || MODE == kColEFourier) {
+    if (MODE == kColEFourier || a.objective == 0) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) ampv[q] = act ? a.ampx[base + out_pos<L, E>(t, q) * st] : 0.f;
+    }
+  }
+  stage_twiddles<L, E, NT>(tws, a.tw);
+  const float2* twl = reinterpret_cast<const float2*>(tws);
+  if constexpr (MODE == kColInv) fft_line<L, E, +1>(v, va, vb, t, twl);
+  else fft_line<L, E, -1>(v, va, vb, t, twl);
 
   if constexpr (MODE == kColProject) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + out_pos<L>(t, q) * st;
+      const long idx = base + out_pos<L, E>(t, q) * st;
       float2 G = v[q];
       if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
       if (a.objective == 0) {
-        G = project_pixel(G, a.ampx[idx], a, idx);
+        G = project_pixel(G, ampv[q], a, idx);
       } else if (a.flags[idx] & kMeasured) {
         G = a.phased_amp[idx];  // src/phasing.c:1103-1109
       }
       v[q] = G;
     }
-    out_to_in<L>(v, va, t);
+    out_to_in<L, E>(v, va, t);
     __syncthreads();
-    fft_line<L, +1>(v, va, vb, t, a.tw);
+    fft_line<L, E, +1>(v, va, vb, t, twl);
   } else if constexpr (MODE == kColBlur) {
     // Gaussian transfer function of sp_gaussian_blur (src/image_filter.c:80-91)
     // with dist_to_top_left (src/image_util.c:1012-1032), computed in double
@@ -447,7 +490,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + out_pos<L>(t, q) * st;
+      const long idx = base + out_pos<L, E>(t, q) * st;
       const int ix = (int)(idx % a.nx);
       const long rest = idx / a.nx;
       const int iy = (int)(rest % a.ny);
@@ -460,9 +503,9 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
       const float h = (float)(exp(k0 * ((double)r * (double)r) * s2) / NN);
       v[q] = make_float2(v[q].x * h, v[q].y * h);
     }
-    out_to_in<L>(v, va, t);
+    out_to_in<L, E>(v, va, t);
     __syncthreads();
-    fft_line<L, +1>(v, va, vb, t, a.tw);
+    fft_line<L, E, +1>(v, va, vb, t, twl);
   } else if constexpr (MODE == kColEFourier) {
     // src/phasing.c:879-888: measured pixels add (|F|-|amp|)^2 and amp^2,
     // unmeasured ones |F|^2; FcFo sums |amp| and |F| over measured pixels.
@@ -470,11 +513,11 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       if (!act) continue;
-      const long idx = base + out_pos<L>(t, q) * st;
+      const long idx = base + out_pos<L, E>(t, q) * st;
       const float2 F = v[q];
       if (a.fout) a.fout[idx] = F;
       const float af = cabsf_ref(F);
-      const float ab = a.ampx[idx];
+      const float ab = ampv[q];
       if (__float_as_uint(ab) != kUnmeasuredBits) {
         const double aa = fabs((double)ab);
         const double d = (double)af - aa;
@@ -497,60 +540,87 @@ __global__ __launch_bounds__(ColShape<L>::NT) void k_col(ColArgs a) {
 
 #pragma unroll
   for (int q = 0; q < NR; ++q)
-    if (act) a.W[base + out_pos<L>(t, q) * st] = v[q];
+    if (act) a.W[base + out_pos<L, E>(t, q) * st] = v[q];
 }
 
 // ---------------------------------------------------------------------------
-// Launch helpers (one instantiation per supported length).
+// Launch helpers and the per-length dispatch table.
 // ---------------------------------------------------------------------------
-template <int L>
+// device twiddle table of plan (L, E), built once per process (engine.hip)
+const float2* twiddles(int L, int E);
+
+template <int L, int E>
 struct LengthOps {
-  static void row(int mode, const RowArgs& a, hipStream_t s) {
-    const dim3 grid((a.nlines + RowShape<L>::M - 1) / RowShape<L>::M), blk(RowShape<L>::NT);
+  static const float2* tw() {
+    static const float2* t = twiddles(L, E);
+    return t;
+  }
+  static void row(int mode, const RowArgs& a0, hipStream_t s) {
+    RowArgs a = a0;
+    a.tw = tw();
+    const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
     switch (mode) {
-      case kRowFwd: hipLaunchKernelGGL((k_row<L, kRowFwd>), grid, blk, 0, s, a); break;
-      case kRowInv: hipLaunchKernelGGL((k_row<L, kRowInv>), grid, blk, 0, s, a); break;
-      case kRowAbsFwd: hipLaunchKernelGGL((k_row<L, kRowAbsFwd>), grid, blk, 0, s, a); break;
-      case kRowBlurOut: hipLaunchKernelGGL((k_row<L, kRowBlurOut>), grid, blk, 0, s, a); break;
+      case kRowFwd: hipLaunchKernelGGL((k_row<L, E, kRowFwd>), grid, blk, 0, s, a); break;
+      case kRowInv: hipLaunchKernelGGL((k_row<L, E, kRowInv>), grid, blk, 0, s, a); break;
+      case kRowAbsFwd: hipLaunchKernelGGL((k_row<L, E, kRowAbsFwd>), grid, blk, 0, s, a); break;
+      case kRowBlurOut: hipLaunchKernelGGL((k_row<L, E, kRowBlurOut>), grid, blk, 0, s, a); break;
     }
   }
-  static void update(const UpdateArgs& a, bool dm, hipStream_t s) {
-    const dim3 grid((a.nlines + RowShape<L>::M - 1) / RowShape<L>::M), blk(RowShape<L>::NT);
-    if (dm) hipLaunchKernelGGL((k_row_update<L, true>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_row_update<L, false>), grid, blk, 0, s, a);
+  static void update(const UpdateArgs& a0, bool dm, hipStream_t s) {
+    UpdateArgs a = a0;
+    a.tw = tw();
+    const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
+    if (dm) hipLaunchKernelGGL((k_row_update<L, E, true>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_row_update<L, E, false>), grid, blk, 0, s, a);
   }
   static int col_blocks(const LineGeom& g) {
-    return ((g.ncontig + ColShape<L>::C - 1) / ColShape<L>::C) * g.nouter;
+    return ((g.ncontig + ColShape<L, E>::C - 1) / ColShape<L, E>::C) * g.nouter;
   }
-  static void col(int mode, const ColArgs& a, hipStream_t s) {
-    const dim3 grid(col_blocks(a.geo)), blk(ColShape<L>::NT);
+  static void col(int mode, const ColArgs& a0, hipStream_t s) {
+    ColArgs a = a0;
+    a.tw = tw();
+    const dim3 grid(col_blocks(a.geo)), blk(ColShape<L, E>::NT);
     switch (mode) {
-      case kColFwd: hipLaunchKernelGGL((k_col<L, kColFwd>), grid, blk, 0, s, a); break;
-      case kColInv: hipLaunchKernelGGL((k_col<L, kColInv>), grid, blk, 0, s, a); break;
-      case kColProject: hipLaunchKernelGGL((k_col<L, kColProject>), grid, blk, 0, s, a); break;
-      case kColEFourier: hipLaunchKernelGGL((k_col<L, kColEFourier>), grid, blk, 0, s, a); break;
-      case kColBlur: hipLaunchKernelGGL((k_col<L, kColBlur>), grid, blk, 0, s, a); break;
+      case kColFwd: hipLaunchKernelGGL((k_col<L, E, kColFwd>), grid, blk, 0, s, a); break;
+      case kColInv: hipLaunchKernelGGL((k_col<L, E, kColInv>), grid, blk, 0, s, a); break;
+      case kColProject: hipLaunchKernelGGL((k_col<L, E, kColProject>), grid, blk, 0, s, a); break;
+      case kColEFourier: hipLaunchKernelGGL((k_col<L, E, kColEFourier>), grid, blk, 0, s, a); break;
+      case kColBlur: hipLaunchKernelGGL((k_col<L, E, kColBlur>), grid, blk, 0, s, a); break;
     }
   }
 };
 
-// Dispatch table entry for one length.
-struct LengthEntry {
-  int L;
-  int twtotal;
+// One compiled variant (plan with E elements per thread at the edges).
+struct Variant {
+  int E;
   void (*row)(int, const RowArgs&, hipStream_t);
   void (*update)(const UpdateArgs&, bool, hipStream_t);
   void (*col)(int, const ColArgs&, hipStream_t);
   int (*col_blocks)(const LineGeom&);
 };
 
-template <int L>
-constexpr LengthEntry length_entry() {
-  return LengthEntry{L, PlanOf<L>::P.twtotal, &LengthOps<L>::row, &LengthOps<L>::update,
-                     &LengthOps<L>::col, &LengthOps<L>::col_blocks};
+template <int L, int E>
+constexpr Variant variant() {
+  return Variant{E, &LengthOps<L, E>::row, &LengthOps<L, E>::update, &LengthOps<L, E>::col,
+                 &LengthOps<L, E>::col_blocks};
 }
 
-// Registry implemented by the instantiation units.
-const LengthEntry* find_length(int L);
+// Dispatch table entry for one length: its compiled variants and the ones in
+// use for row passes and for column passes (chosen by the tuner).
+struct LengthEntry {
+  int L;
+  int nvar;
+  Variant var[3];
+  int row_choice;
+  int col_choice;
+  void row(int mode, const RowArgs& a, hipStream_t s) const { var[row_choice].row(mode, a, s); }
+  void update(const UpdateArgs& a, bool dm, hipStream_t s) const { var[row_choice].update(a, dm, s); }
+  void col(int mode, const ColArgs& a, hipStream_t s) const { var[col_choice].col(mode, a, s); }
+  int col_blocks(const LineGeom& g) const { return var[col_choice].col_blocks(g); }
+};
+
+// Registry implemented by the instantiation units (mutable: the tuner sets
+// the choices).
+LengthEntry* find_length(int L);
 
 }  // namespace spx
