@@ -45,8 +45,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-reconstruction workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served workload")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only launch 20 plain 64 MiB device-to-device copies (counter calibration)")
     return ap.parse_args()
 
 
@@ -142,6 +146,153 @@ def gpu_bench(args):
     return t_iter, roofline, iteration_roofline, recon
 
 
+def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=2, seed=2000):
+    """Many-reconstruction workload: n_patterns seeded disc patterns with sizes
+    drawn from {128, 256, 384, 512}^2, n_starts random-phase starts each, HIO
+    beta 0.9 for `iters` iterations with an area shrinkwrap every `period`
+    (blur 3.0 -> 0.7 px over the run, 8 % of the pixels).
+
+    Batched: one SpxBatch per size class advances all its reconstructions
+    together (HIP-graph replay between support updates).  Plain: the
+    reference-style loop, one SpPhaser per reconstruction, timed on
+    `plain_per_size` reconstructions per size class from the same starts and
+    extrapolated to the size class's count.  Inputs are uploaded before the
+    clock starts in both forms; the clock stops after a synchronising
+    efourier read-back."""
+    import numpy as np
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import workloads
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 256, 384, 512], n_patterns)
+    classes = {}
+    for i, n in enumerate(sizes):
+        classes.setdefault(int(n), []).append(workloads.disc_scaled(int(n), seed=seed + 1 + i))
+    keep = []
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    bmap, amap = sp.smap([(0, 3.0), (iters, 0.7)]), sp.smap([(0, 0.08)])
+    keep += [bmap, amap]
+    sarr = sp.sp_support_array_init(sp.sp_support_area_alloc(bmap, amap), period)
+    t_batch = t_plain = 0.0
+    per_size, worst = {}, 0.0
+    for n, pats in sorted(classes.items()):
+        nrec = len(pats) * n_starts
+        b = sp.Batch((n, n), nrec)
+        for j, p in enumerate(pats):
+            for s in range(n_starts):
+                b.set_data(j * n_starts + s, p.amplitudes, p.mask, p.support)
+        b.init_model(sp.SpModelRandomPhases, seed + n)
+        b.set_algorithm(hio)
+        b.set_support_schedule(sarr)
+        b.use_graphs(True)
+        starts = [b.model(r) for r in range(plain_per_size)]
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.iterate(iters)
+        ef_b, _ = b.efourier()
+        tb = time.perf_counter() - t0
+        b.close()
+        # plain loop over the first reconstructions of the class
+        tp, ef_p = 0.0, []
+        for r in range(plain_per_size):
+            p = pats[r // n_starts]
+            ph = sp.sp_phaser_alloc()
+            if sp.sp_phaser_init(ph, hio, sarr, sp.SpEngineHIP) != 0:
+                raise RuntimeError(sp.last_error())
+            imgs = [sp.image_from_array(p.amplitudes, p.mask), sp.image_from_array(starts[r]),
+                    sp.image_from_array(p.support)]
+            sp.sp_phaser_set_amplitudes(ph, imgs[0])
+            if sp.sp_phaser_init_model(ph, imgs[1], 0) != 0 or sp.sp_phaser_init_support(ph, imgs[2], 0, 0.0) != 0:
+                raise RuntimeError(sp.last_error())
+            sp.sp_phaser_efourier(ph, None)  # synchronises the uploads
+            t0 = time.perf_counter()
+            sp.sp_phaser_iterate(ph, iters)
+            ef = sp.sp_phaser_efourier(ph, None)
+            tp += time.perf_counter() - t0
+            ef_p.append(ef)
+            sp.sp_phaser_free(ph)
+        tp_class = tp / plain_per_size * nrec
+        diff = max(abs(float(ef_b[r]) - e) / max(abs(e), 1e-30) for r, e in enumerate(ef_p))
+        worst = max(worst, diff)
+        t_batch += tb
+        t_plain += tp_class
+        per_size[f"{n}x{n}"] = {"recons": nrec, "batched_s": tb, "plain_s_extrapolated": tp_class,
+                                "mean_efourier": float(np.mean(ef_b))}
+    total = n_patterns * n_starts
+    return {"workload": f"{n_patterns} patterns x {n_starts} random starts, sizes {{128,256,384,512}}^2, "
+                        f"HIO beta 0.9, {iters} iterations, area shrinkwrap every {period}",
+            "recons": total, "batched_seconds": t_batch, "batched_recons_per_s": total / t_batch,
+            "plain_seconds_extrapolated": t_plain, "plain_recons_per_s": total / t_plain,
+            "speedup": t_plain / t_batch, "plain_sample": f"{plain_per_size} reconstructions per size class",
+            "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
+
+
+def _serving_pattern(args):
+    from libspimage_amd import workloads
+    shape, seed = args
+    return workloads.serving_pattern(shape, seed)
+
+
+def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
+    """Served workload: a Poisson trace at `rate` requests/s for `seconds`
+    (85 % 2-D n x n with n FFT-smooth in [128, 1024], 15 % 3-D n^3 with n in
+    {64, 80, 96, 128}; ER / HIO (beta 0.9) / RAAR (beta 0.87); 100-1000
+    iterations) through the continuous-batching server, in real time and
+    with every request queued at once (capacity), and through the plain
+    reference-style loop (one SpPhaser per request, in arrival order)."""
+    from concurrent.futures import ProcessPoolExecutor
+    from multiprocessing import get_context
+    import numpy as np
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import serve, workloads
+    sizes = [n for n in workloads.smooth_sizes(128, 1024) if sp.spx_supported_length(n)]
+    trace = workloads.serving_trace(rate, seconds, seed, sizes_2d=sizes)
+    shapes = sorted({s for _, s, _, _, _ in trace})
+    with ProcessPoolExecutor(workers, mp_context=get_context("spawn")) as ex:
+        pats = dict(zip(shapes, ex.map(_serving_pattern, [(s, seed + i) for i, s in enumerate(shapes)])))
+
+    def requests(realtime=True):
+        return [serve.Request(k, pats[s].amplitudes, pats[s].mask, pats[s].support, alg, its, sd,
+                              beta=0.9 if alg == "hio" else 0.87, arrival=t if realtime else 0.0)
+                for k, (t, s, alg, its, sd) in enumerate(trace)]
+
+    # warm-up: every shape once through both paths (per-length tuning, twiddles)
+    warm = [serve.Request(-1 - i, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
+            for i, s in enumerate(shapes)]
+    plain = serve.PlainRunner()
+    for r in warm:
+        plain.run_one(r)
+    srv = serve.Server()
+    srv.run(warm, realtime=False)
+    srv.close()
+
+    srv = serve.Server()
+    t0 = time.perf_counter()
+    live = srv.run(requests(True), realtime=True)
+    t_live = time.perf_counter() - t0
+    created_live = srv.batches_created
+    srv.close()
+    srv = serve.Server()
+    t0 = time.perf_counter()
+    offline = srv.run(requests(False), realtime=False)
+    t_off = time.perf_counter() - t0
+    srv.close()
+    plain_res, t_plain = plain.run(requests(True))
+    ef_plain = {r.rid: r.efourier for r in plain_res}
+    diff = max(abs(r.efourier - ef_plain[r.rid]) / max(abs(ef_plain[r.rid]), 1e-30) for r in live)
+    n = len(trace)
+    return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s ({n} requests): 85 % 2-D n^2, n FFT-smooth in "
+                        "[128, 1024]; 15 % 3-D n^3, n in {64, 80, 96, 128}; ER / HIO 0.9 / RAAR 0.87 uniform; "
+                        "100-1000 iterations; static support; random-phase starts",
+            "requests": n, "shapes": len(shapes),
+            "served": {"latency": serve.latency_summary(live), "makespan_s": t_live, "recons_per_s": n / t_live,
+                       "batches_created": created_live},
+            "capacity_all_queued": {"seconds": t_off, "recons_per_s": n / t_off},
+            "plain_loop": {"seconds": t_plain, "recons_per_s": n / t_plain,
+                           "latency_fifo": serve.latency_summary(plain_res)},
+            "capacity_speedup": t_plain / t_off,
+            "efourier_max_rel_diff_served_vs_plain": diff}
+
+
 def cpu_baseline_worker(seconds: float):
     """Runs in a child process: the oracle's HIO loop on one core."""
     from libspimage_amd import workloads
@@ -178,8 +329,24 @@ def cpu_baseline(seconds: float):
                       f"(oracle restatement of src/phasing.c:1142-1172, FFT: {d['backend']})"}
 
 
+def copy_calibration():
+    """20 launches of torch's copy kernel over 64 MiB: the known byte count
+    against which HBM read/write counters are scaled."""
+    import torch
+    n = 64 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(20):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 20}))
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.cpu_baseline_only:
         cpu_baseline_worker(args.cpu_seconds)
         return
@@ -188,6 +355,8 @@ def main():
               "(independent reconstructions shard by id across GPUs with no exchange)", file=sys.stderr)
         raise SystemExit(2)
     t_iter, roofline, it_roof, recon = gpu_bench(args)
+    batched = None if args.no_batched else batched_bench()
+    serving = None if args.no_serving else serving_bench()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
@@ -207,6 +376,8 @@ def main():
         "roofline": roofline,
         "iteration_roofline": it_roof,
         "reconstruction_1024": recon,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

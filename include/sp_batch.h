@@ -39,8 +39,12 @@ spimage_EXPORT int spx_batch_set_support(SpxBatch *b, int rec, const float *supp
 spimage_EXPORT int spx_batch_set_model(SpxBatch *b, int rec, const float *model_complex);
 /* SpModelRandomPhases / SpModelZeroPhases / SpModelRandomValues start for all
  * reconstructions from their amplitudes (reference src/phasing.c:626-679);
- * the generator stream of reconstruction r differs from every other one */
+ * reconstruction r draws from the generator seeded with seed + r */
 spimage_EXPORT int spx_batch_init_model(SpxBatch *b, int flags, unsigned long long seed);
+/* The same start for reconstruction `rec` only, seeded with `seed`: equal to
+ * a single phaser's sp_phaser_init_model(ph, NULL, flags) after
+ * spx_phaser_set_seed(ph, seed).  The other reconstructions keep their state. */
+spimage_EXPORT int spx_batch_init_model_rec(SpxBatch *b, int rec, int flags, unsigned long long seed);
 
 /* algorithm of all reconstructions (the reference's allocators build it) */
 spimage_EXPORT int spx_batch_set_algorithm(SpxBatch *b, SpPhasingAlgorithm *alg);
@@ -67,6 +71,13 @@ spimage_EXPORT int spx_batch_get_support(SpxBatch *b, int rec, int *support);
 spimage_EXPORT int spx_batch_time_iterations(SpxBatch *b, int iterations, float *ms);
 /* wait for all queued work of the batch */
 spimage_EXPORT int spx_batch_synchronize(SpxBatch *b);
+/* 1 when all work queued for the batch has finished, 0 while some is still
+ * running (never blocks) */
+spimage_EXPORT int spx_batch_query(SpxBatch *b);
+/* Device bytes a batch of nrec reconstructions of (nz, ny, nx) allocates for
+ * ER / HIO / RAAR with area or threshold support updates (the difference
+ * map adds 8 bytes per pixel); used to admit work by free device memory. */
+spimage_EXPORT long long spx_batch_estimate_bytes(int nx, int ny, int nz, int nrec);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,9 @@ extern "C" {
 
 /* 1 when a HIP device is usable, 0 otherwise */
 spimage_EXPORT int spx_device_available(void);
+/* Free and total bytes of device memory on the library's device
+ * (hipMemGetInfo). */
+spimage_EXPORT int spx_device_memory(unsigned long long *free_bytes, unsigned long long *total_bytes);
 spimage_EXPORT const char *spx_last_error(void);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);

@@ -966,6 +966,22 @@ Image* sp_gaussian_blur(Image* in, real radius) {
 // ---------------------------------------------------------------------------
 int spx_device_available(void) { return spx::device_available() ? 1 : 0; }
 const char* spx_last_error(void) { return spx::last_error(); }
+int spx_device_memory(unsigned long long* free_b, unsigned long long* total_b) {
+  if (!spx::device_available()) {
+    spx::set_error("no HIP device available");
+    return -10;
+  }
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) {
+    spx::set_error("hipMemGetInfo: %s", hipGetErrorString(e));
+    return -20;
+  }
+  if (free_b) *free_b = f;
+  if (total_b) *total_b = t;
+  return 0;
+}
+
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
 int spx_fft_variant(int L, int* row_e, int* col_e) {

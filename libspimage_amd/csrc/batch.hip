@@ -161,6 +161,29 @@ int spx_batch_init_model(SpxBatch* b, int flags, unsigned long long seed) {
   return rc;
 }
 
+int spx_batch_init_model_rec(SpxBatch* b, int rec, int flags, unsigned long long seed) {
+  if (!b || rec < 0 || rec >= b->nrec) return fail(-1, "bad batch arguments");
+  int rc = b->e->init_model_from_amplitudes(b->amp.data() + (size_t)rec * b->N, flags, seed, rec);
+  if (!rc) rc = b->e->finish_model_init(flags, rec);
+  return rc;
+}
+
+int spx_batch_query(SpxBatch* b) {
+  if (!b) return fail(-1, "bad batch arguments");
+  const hipError_t e = hipStreamQuery(b->e->s);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(-20, hipGetErrorString(e));
+}
+
+long long spx_batch_estimate_bytes(int nx, int ny, int nz, int nrec) {
+  // five complex64 arrays (two iterates, the row-transformed iterate, the
+  // model before projection, scratch), two float arrays (encoded amplitudes,
+  // |blur|), one flag byte, and the per-reconstruction select scratch
+  const long long N = (long long)nx * ny * nz;
+  return (long long)nrec * (N * (5 * 8 + 2 * 4 + 1) + (long long)(spx::kHistStride + spx::kCtlStride) * 4 + 4096);
+}
+
 int spx_batch_set_algorithm(SpxBatch* b, SpPhasingAlgorithm* alg) {
   if (!b || !alg) return fail(-2, "Algorithm is NULL!");
   if (alg->type != SpHIO && alg->type != SpRAAR && alg->type != SpER && alg->type != SpDiffMap)

@@ -83,9 +83,14 @@ class Engine {
   int set_rec_params(const int* alg, const float* beta);
 
   // ---- model initialisation (reference src/phasing.c:606-724) ----
-  int init_model_from_amplitudes(const float* amp_raw, int flags, uint64_t seed);
+  // rec < 0: every reconstruction, amp_raw holds NT values and reconstruction
+  // r draws from seed + r; rec >= 0: that reconstruction only, amp_raw holds
+  // its N values and it draws from seed.  The generator is indexed by the
+  // pixel's position inside its reconstruction, so a reconstruction seeded
+  // with s starts the same in any batch and alone.
+  int init_model_from_amplitudes(const float* amp_raw, int flags, uint64_t seed, int rec = -1);
   int init_model_user(const float2* model, int flags);
-  int finish_model_init(int flags);  // masked-out zeroing, g0 = gp = 0
+  int finish_model_init(int flags, int rec = -1);  // masked-out zeroing, g0 = gp = 0
 
   // ---- iterations ----
   // Runs n iterations with betas[i] (ignored when per-reconstruction betas
@@ -128,10 +133,11 @@ class Engine {
   int iterate_impl(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks);
   int ensure_R();
   int ensure_Rf(float gamma1);
-  int reduce_per_rec(int parts_per_rec, int nv, double* host_out);
+  int reduce_per_rec(int parts_per_rec, int nv, double* host_out, int count = -1);
   int blur_to_abs(float sigma, bool hist);
 
   Shape sh{};
+  Shape one{};  // the same shape with one reconstruction (per-slot transforms)
   float2* g[2] = {nullptr, nullptr};
   int cur = 0;
   float2 *W = nullptr, *Wf = nullptr, *gp = nullptr, *S1 = nullptr;

@@ -228,11 +228,14 @@ __device__ __forceinline__ double rng_u01(uint64_t seed, uint64_t i, uint64_t st
 
 // |amp| with a random (or zero) phase: sp_image_rephase (reference
 // src/image_util.c:1044-1067)
-__global__ void k_init_phases(const float* amp, float2* out, long n, uint64_t seed, int random) {
+// n_per_rec pixels per reconstruction; reconstruction r of the range draws
+// from seed + r at its local pixel index.
+__global__ void k_init_phases(const float* amp, float2* out, long n_per_rec, long n, uint64_t seed, int random) {
   GRID_STRIDE(i, n) {
     const float a = fabsf(amp[i]);
     if (random) {
-      const float phase = (float)(rng_u01(seed, i, 0) * M_PI * 2.0);
+      const long r = i / n_per_rec;
+      const float phase = (float)(rng_u01(seed + (uint64_t)r, i - r * n_per_rec, 0) * M_PI * 2.0);
       out[i] = make_float2((float)(cos((double)phase) * a), (float)(sin((double)phase) * a));
     } else {
       out[i] = make_float2(a, 0.f);
@@ -240,9 +243,12 @@ __global__ void k_init_phases(const float* amp, float2* out, long n, uint64_t se
   }
 }
 
-__global__ void k_random_values(float2* out, long n, uint64_t seed) {
-  GRID_STRIDE(i, n) out[i] = make_float2((float)((float)rng_u01(seed, i, 0) - 0.5),
-                                         (float)((float)rng_u01(seed, i, 1) - 0.5));
+__global__ void k_random_values(float2* out, long n_per_rec, long n, uint64_t seed) {
+  GRID_STRIDE(i, n) {
+    const long r = i / n_per_rec, j = i - r * n_per_rec;
+    out[i] = make_float2((float)((float)rng_u01(seed + (uint64_t)r, j, 0) - 0.5),
+                         (float)((float)rng_u01(seed + (uint64_t)r, j, 1) - 0.5));
+  }
 }
 
 __global__ void k_amp2(const float* amp, float2* out, long n) {
@@ -556,14 +562,15 @@ Engine* Engine::create(int nx, int ny, int nz, int nrec) {
     set_error("no HIP device available");
     return nullptr;
   }
-  Shape sh;
-  if (make_shape(nx, ny, nz, nrec, &sh)) return nullptr;
+  Shape sh, one;
+  if (make_shape(nx, ny, nz, nrec, &sh) || make_shape(nx, ny, nz, 1, &one)) return nullptr;
   if (!sh.ex || (!sh.ey && !sh.ez)) {
     set_error("the HIP engine phases 2-D and 3-D arrays with nx > 1 (got %d x %d x %d)", nx, ny, nz);
     return nullptr;
   }
   Engine* e = new Engine();
   e->sh = sh;
+  e->one = one;
   e->nx = nx;
   e->ny = ny;
   e->nz = nz;
@@ -684,8 +691,10 @@ int Engine::download(const float2* dev, float2* host, int rec) {
   return 0;
 }
 
+// Captured graphs read the tables at run time, so new contents need no
+// recapture; only a table's first allocation changes the kernel arguments.
 int Engine::set_rec_params(const int* alg, const float* beta) {
-  drop_graphs();
+  if ((alg && !alg_rec) || (beta && !beta_rec)) drop_graphs();
   if (alg) {
     if (!alg_rec) SPX_HIP(hipMalloc(&alg_rec, sizeof(int) * nrec));
     SPX_HIP(hipMemcpyAsync(alg_rec, alg, sizeof(int) * nrec, hipMemcpyHostToDevice, s));
@@ -704,35 +713,40 @@ int Engine::fft_nd(const float2* in, float2* out, int dir) {
   return 0;
 }
 
-int Engine::reduce_per_rec(int parts, int nv, double* host_out) {
-  hipLaunchKernelGGL(k_reduce_rec, dim3(nrec), dim3(256), 0, s, partials, parts, nv, red);
+int Engine::reduce_per_rec(int parts, int nv, double* host_out, int count) {
+  const int nr = count < 0 ? nrec : count;
+  hipLaunchKernelGGL(k_reduce_rec, dim3(nr), dim3(256), 0, s, partials, parts, nv, red);
   SPX_HIP(hipGetLastError());
-  SPX_HIP(hipMemcpyAsync(host_out, red, sizeof(double) * nv * nrec, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipMemcpyAsync(host_out, red, sizeof(double) * nv * nr, hipMemcpyDeviceToHost, s));
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
-int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd) {
-  SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd, int rec) {
+  const int cnt = rec < 0 ? nrec : 1;
+  const long off = rec < 0 ? 0 : rec * N, n = cnt * N;
+  const Shape& shp = rec < 0 ? sh : one;
+  SPX_HIP(hipMemcpyAsync(absb + off, amp_raw, sizeof(float) * n, hipMemcpyHostToDevice, s));
   const float inv_n = (float)(1.0 / (double)N);
   if (fl & 1 /* SpModelRandomPhases */ || fl & 2 /* SpModelZeroPhases */) {
-    hipLaunchKernelGGL(k_init_phases, ew_grid(NT), dim3(256), 0, s, absb, S1, NT, sd, (fl & 1) ? 1 : 0);
-    if (fft_nd(S1, S1, +1)) return -20;
-    hipLaunchKernelGGL(k_scale_copy, ew_grid(NT), dim3(256), 0, s, S1, g[cur], NT, inv_n);
+    hipLaunchKernelGGL(k_init_phases, ew_grid(n), dim3(256), 0, s, absb + off, S1 + off, N, n, sd,
+                       (fl & 1) ? 1 : 0);
+    enqueue_fft(shp, S1 + off, S1 + off, +1, s);
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(n), dim3(256), 0, s, S1 + off, g[cur] + off, n, inv_n);
   } else if (fl & 4 /* SpModelRandomValues */) {
     // reference src/phasing.c:666-679: scale so that sum |g|^2 = sum amp^2 / N
     const int bpr = rec_blocks(N);
-    std::vector<double> amp2((size_t)nrec), mod2((size_t)nrec);
-    hipLaunchKernelGGL(k_sumsq_f32, dim3(bpr, nrec), dim3(kRedThreads), 0, s, absb, N, partials);
-    if (reduce_per_rec(bpr, 1, amp2.data())) return -20;
-    hipLaunchKernelGGL(k_random_values, ew_grid(NT), dim3(256), 0, s, g[cur], NT, sd);
-    hipLaunchKernelGGL(k_sumsq_c64, dim3(bpr, nrec), dim3(kRedThreads), 0, s, g[cur], N, partials);
-    if (reduce_per_rec(bpr, 1, mod2.data())) return -20;
-    std::vector<float> sc((size_t)nrec);
-    for (int r = 0; r < nrec; ++r) sc[r] = (float)sqrt((amp2[r] / (double)N) / mod2[r]);
+    std::vector<double> amp2((size_t)cnt), mod2((size_t)cnt);
+    hipLaunchKernelGGL(k_sumsq_f32, dim3(bpr, cnt), dim3(kRedThreads), 0, s, absb + off, N, partials);
+    if (reduce_per_rec(bpr, 1, amp2.data(), cnt)) return -20;
+    hipLaunchKernelGGL(k_random_values, ew_grid(n), dim3(256), 0, s, g[cur] + off, N, n, sd);
+    hipLaunchKernelGGL(k_sumsq_c64, dim3(bpr, cnt), dim3(kRedThreads), 0, s, g[cur] + off, N, partials);
+    if (reduce_per_rec(bpr, 1, mod2.data(), cnt)) return -20;
+    std::vector<float> sc((size_t)cnt);
+    for (int r = 0; r < cnt; ++r) sc[r] = (float)sqrt((amp2[r] / (double)N) / mod2[r]);
     float* dsc = (float*)partials;  // free again: the reductions above are done
-    SPX_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * nrec, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_scale_rec, ew_grid(NT), dim3(256), 0, s, g[cur], N, NT, dsc);
+    SPX_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * cnt, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scale_rec, ew_grid(n), dim3(256), 0, s, g[cur] + off, N, n, dsc);
     SPX_HIP(hipStreamSynchronize(s));
   } else {
     return -3;
@@ -746,15 +760,18 @@ int Engine::init_model_user(const float2* model, int) {
   return 0;
 }
 
-int Engine::finish_model_init(int fl) {
+int Engine::finish_model_init(int fl, int rec) {
+  const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
+  const Shape& shp = rec < 0 ? sh : one;
   if (fl & 256 /* SpModelMaskedOutZeroed */) {
-    if (fft_nd(g[cur], S1, -1)) return -20;
-    hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(NT), dim3(256), 0, s, S1, flags, NT);
-    if (fft_nd(S1, S1, +1)) return -20;
-    hipLaunchKernelGGL(k_scale_copy, ew_grid(NT), dim3(256), 0, s, S1, g[cur], NT, (float)(1.0 / (double)N));
+    enqueue_fft(shp, g[cur] + off, S1 + off, -1, s);
+    hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(n), dim3(256), 0, s, S1 + off, flags + off, n);
+    enqueue_fft(shp, S1 + off, S1 + off, +1, s);
+    hipLaunchKernelGGL(k_scale_copy, ew_grid(n), dim3(256), 0, s, S1 + off, g[cur] + off, n,
+                       (float)(1.0 / (double)N));
   }
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, g[cur ^ 1], NT, make_float2(0.f, 0.f));
-  hipLaunchKernelGGL(k_fill_c64, ew_grid(NT), dim3(256), 0, s, gp, NT, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(n), dim3(256), 0, s, g[cur ^ 1] + off, n, make_float2(0.f, 0.f));
+  hipLaunchKernelGGL(k_fill_c64, ew_grid(n), dim3(256), 0, s, gp + off, n, make_float2(0.f, 0.f));
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   invalidate_R();

@@ -279,6 +279,18 @@ _bind("spx_batch_get_model", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_get_support", C.c_int, PBatch, C.c_int, PI)
 _bind("spx_batch_time_iterations", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_synchronize", C.c_int, PBatch)
+_bind("spx_batch_init_model_rec", C.c_int, PBatch, C.c_int, C.c_int, C.c_ulonglong)
+_bind("spx_batch_query", C.c_int, PBatch)
+_bind("spx_batch_estimate_bytes", C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int)
+_bind("spx_device_memory", C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
+
+
+def device_memory() -> tuple[int, int]:
+    """(free, total) bytes of device memory."""
+    f, t = C.c_ulonglong(), C.c_ulonglong()
+    if spx_device_memory(C.byref(f), C.byref(t)) != 0:
+        raise RuntimeError(last_error())
+    return f.value, t.value
 
 
 def _fptr(a):
@@ -390,6 +402,25 @@ class Batch:
 
     def synchronize(self):
         self._check(spx_batch_synchronize(self.h), "synchronize")
+
+    def init_model_rec(self, rec, flags, seed):
+        self._check(spx_batch_init_model_rec(self.h, rec, flags, seed), "init_model_rec")
+
+    def done(self) -> bool:
+        """True when the queued work has finished (never blocks)."""
+        r = spx_batch_query(self.h)
+        if r < 0:
+            raise RuntimeError(f"spx_batch_query failed: {last_error()}")
+        return r == 1
+
+    def set_amplitudes(self, rec, amplitudes, mask):
+        a = np.ascontiguousarray(amplitudes, np.float32)
+        m = np.ascontiguousarray(mask, np.int32)
+        self._check(spx_batch_set_amplitudes(self.h, rec, _fptr(a), _iptr(m)), "set_amplitudes")
+
+    def set_support(self, rec, support):
+        s = np.ascontiguousarray(support, np.float32)
+        self._check(spx_batch_set_support(self.h, rec, _fptr(s)), "set_support")
 
 
 # ---------------------------------------------------------------------------

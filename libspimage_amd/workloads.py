@@ -129,3 +129,57 @@ def disc_scaled(n: int, seed: int) -> Pattern:
     f = n / 256.0
     return make_pattern((n, n), n_gauss=max(4, int(round(20 * f))), sigma=(2, 6), radius=40 * f,
                         photons=1e5 * f * f, beamstop=4, dilate=3, seed=seed)
+
+
+def smooth_sizes(lo: int = 128, hi: int = 1024) -> list[int]:
+    """FFT-smooth lengths 2^a 3^b 5^c in [lo, hi]."""
+    out = set()
+    p2 = 1
+    while p2 <= hi:
+        p3 = p2
+        while p3 <= hi:
+            p5 = p3
+            while p5 <= hi:
+                if p5 >= lo:
+                    out.add(p5)
+                p5 *= 5
+            p3 *= 3
+        p2 *= 2
+    return sorted(out)
+
+
+def serving_pattern(shape, seed: int) -> Pattern:
+    """Light pattern for a serving request of any shape: 10 Gaussians in a
+    disc / ball of radius n/6.4 (40 px at 256), 1e5 photons per 256^2 of
+    area, beamstop 4, support = region + 3 px."""
+    n = min(shape)
+    f = n / 256.0
+    return make_pattern(shape, n_gauss=10, sigma=(2, 6), radius=40 * f, photons=1e5 * f * f * (n if len(shape) == 3 else 1),
+                        beamstop=4, dilate=3, seed=seed)
+
+
+def serving_trace(rate: float = 200.0, seconds: float = 3.0, seed: int = 4000, sizes_2d=None, sizes_3d=(64, 80, 96, 128),
+                  frac_3d: float = 0.15, its=(100, 1000), quantum: int = 50):
+    """Poisson request trace: (arrival, shape, algorithm, iterations, seed)
+    tuples.  85 % 2-D shapes n x n with n drawn from `sizes_2d` (default the
+    FFT-smooth lengths in [128, 1024]), 15 % 3-D n^3 from `sizes_3d`;
+    algorithm uniform over ER / HIO / RAAR; iterations uniform over the
+    multiples of `quantum` in `its`."""
+    rng = np.random.default_rng(seed)
+    sizes_2d = list(sizes_2d) if sizes_2d is not None else smooth_sizes(128, 1024)
+    t, out, k = 0.0, [], 0
+    iters = np.arange(its[0], its[1] + 1, quantum)
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        if rng.uniform() < frac_3d:
+            n = int(rng.choice(sizes_3d))
+            shape = (n, n, n)
+        else:
+            n = int(rng.choice(sizes_2d))
+            shape = (n, n)
+        alg = ("er", "hio", "raar")[int(rng.integers(3))]
+        out.append((t, shape, alg, int(rng.choice(iters)), seed * 1000 + k))
+        k += 1
+    return out

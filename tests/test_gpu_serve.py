@@ -1,0 +1,96 @@
+"""The continuous-batching server (libspimage_amd/serve.py) against requests
+run alone through the reference-style phaser API.
+
+A request served in a mix of other requests (other shapes, algorithms,
+iteration counts, slots reused by earlier requests, batches created while
+others run) must give the same model bit for bit, and the same Fourier
+error, as the same request run alone with sp_phaser_iterate."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pattern(shape, seed):
+    from libspimage_amd import workloads
+    return workloads.make_pattern(shape, n_gauss=5, radius=tuple(s / 6 for s in shape), photons=1e6, beamstop=2,
+                                  seed=seed)
+
+
+def _requests(shapes_algs_its, seed0=500):
+    from libspimage_amd import serve
+    pats = {}
+    out = []
+    for k, (shape, alg, its) in enumerate(shapes_algs_its):
+        if shape not in pats:
+            pats[shape] = _pattern(shape, seed0 + len(pats))
+        p = pats[shape]
+        out.append(serve.Request(k, p.amplitudes, p.mask, p.support, alg, its, seed=seed0 * 100 + k,
+                                 beta=0.9 if alg == "hio" else 0.87, arrival=0.002 * k))
+    return out
+
+
+MIX = [((64, 64), "hio", 100), ((48, 48), "er", 50), ((64, 64), "raar", 150), ((16, 16, 16), "hio", 100),
+       ((64, 64), "er", 50), ((48, 48), "hio", 150), ((64, 64), "hio", 100), ((16, 16, 16), "raar", 50),
+       ((64, 64), "raar", 100), ((48, 48), "raar", 100), ((64, 64), "hio", 50), ((16, 16, 16), "er", 150)]
+
+
+@pytest.mark.parametrize("realtime", [False, True])
+def test_served_equals_alone(sp, realtime):
+    from libspimage_amd import serve
+    reqs = _requests(MIX)
+    # at most 2 slots per 64x64 batch: later requests reuse freed slots and
+    # extra batches are created while others run
+    srv = serve.Server(quantum=50, max_pixels_per_batch=2 * 64 * 64, reserve_bytes=0, keep_models=True)
+    res = srv.run(reqs, realtime=realtime)
+    assert sorted(r.rid for r in res) == list(range(len(reqs)))
+    assert srv.batches_created >= 4
+    srv.close()
+    plain = serve.PlainRunner()
+    by_id = {r.rid: r for r in res}
+    for req in reqs:
+        ef, fc, model = plain.run_one(req, keep_model=True)
+        got = by_id[req.rid]
+        assert np.array_equal(got.model, model), req.rid
+        assert abs(got.efourier - ef) <= 1e-6 * abs(ef), req.rid
+        assert abs(got.fcfo - fc) <= 1e-6 * abs(fc), req.rid
+        assert got.finished >= got.admitted >= (got.arrival if realtime else 0.0)
+
+
+def test_batch_slot_start_equals_single_seeded_start(sp):
+    """spx_batch_init_model_rec(rec, seed) == spx_phaser_set_seed(seed) +
+    sp_phaser_init_model(NULL, SpModelRandomPhases)."""
+    from libspimage_amd import serve
+    reqs = _requests([((40, 40), "hio", 50)] * 3)
+    b = sp.Batch((40, 40), 3)
+    for r, q in enumerate(reqs):
+        b.set_data(r, q.amplitudes, q.mask, q.support)
+        b.init_model_rec(r, sp.SpModelRandomPhases, q.seed)
+    m = b.model()
+    b.close()
+    plain = serve.PlainRunner()
+    for r, q in enumerate(reqs):
+        ph = sp.sp_phaser_alloc()
+        assert sp.sp_phaser_init(ph, plain._alg(q), None, sp.SpEngineHIP) == 0
+        img = sp.image_from_array(q.amplitudes, q.mask)
+        sp.sp_phaser_set_amplitudes(ph, img)
+        sp.spx_phaser_set_seed(ph, q.seed)
+        assert sp.sp_phaser_init_model(ph, None, sp.SpModelRandomPhases) == 0
+        single = sp.image_view(sp.sp_phaser_model(ph)).copy()
+        sp.sp_phaser_free(ph)
+        sp.sp_image_free(img)
+        assert np.array_equal(m[r], single), r
+    assert not np.array_equal(m[0], m[1])
+
+
+def test_admission_respects_free_memory(sp):
+    """With a reserve larger than the free device memory nothing can be
+    admitted; the capacity rule reports 0 slots instead of allocating."""
+    from libspimage_amd import serve
+    free, total = sp.device_memory()
+    assert 0 < free <= total
+    srv = serve.Server(reserve_bytes=total + 1)
+    assert srv._capacity((64, 64), 4) == 0
+    srv2 = serve.Server(reserve_bytes=0, max_pixels_per_batch=1 << 20)
+    assert srv2._capacity((64, 64), 5) == 8
+    assert srv2._capacity((64, 64), 1000) == 256
