@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-reconstruction workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the served workload")
+    ap.add_argument("--no-volume", action="store_true", help="skip the 256^3 reconstruction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -107,8 +108,9 @@ def gpu_bench(args):
     tfile = ROOT / "profiles" / "hio_1024_hbm_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get(dom_name)
-        except (ValueError, OSError):
+            entry = json.loads(tfile.read_text()).get(dom_name)
+            traffic = entry["bytes"] if isinstance(entry, dict) else entry
+        except (ValueError, OSError, KeyError, TypeError):
             traffic = None
     roofline = {
         "bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -224,6 +226,68 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             "plain_seconds_extrapolated": t_plain, "plain_recons_per_s": total / t_plain,
             "speedup": t_plain / t_batch, "plain_sample": f"{plain_per_size} reconstructions per size class",
             "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
+
+
+def volume_bench(dm_its=200, er_its=100, period=20):
+    """3-D reconstruction of the 256^3 ellipsoid pattern: difference map
+    (beta 0.8, gamma1 = gamma2 = infinity -> Elser's values) for `dm_its`
+    iterations, then ER for `er_its`, with an area shrinkwrap every `period`
+    (blur 3.0 -> 1.0 voxels over the run, 4 % of the voxels).  Also the
+    device time of plain DM and ER iterations (no support updates) against
+    their algorithmic bytes (3-D: DM 211 N, ER 102 N)."""
+    import math
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import workloads
+    p = workloads.ellipsoid_256()
+    N = p.amplitudes.size
+    total = dm_its + er_its
+    beta = sp.smap([(0, 0.8)])
+    dm = sp.sp_phasing_diff_map_alloc(beta, math.inf, math.inf, 0)
+    er = sp.sp_phasing_er_alloc(0)
+    bmap, amap = sp.smap([(0, 3.0), (total, 1.0)]), sp.smap([(0, 0.04)])
+    sarr = sp.sp_support_array_init(sp.sp_support_area_alloc(bmap, amap), period)
+
+    def phaser(alg, sup):
+        ph = sp.sp_phaser_alloc()
+        if sp.sp_phaser_init(ph, alg, sup, sp.SpEngineHIP) != 0:
+            raise RuntimeError(sp.last_error())
+        imgs = [sp.image_from_array(p.amplitudes, p.mask), sp.image_from_array(p.start),
+                sp.image_from_array(p.support)]
+        sp.sp_phaser_set_amplitudes(ph, imgs[0])
+        if sp.sp_phaser_init_model(ph, imgs[1], 0) != 0 or sp.sp_phaser_init_support(ph, imgs[2], 0, 0.0) != 0:
+            raise RuntimeError(sp.last_error())
+        for im in imgs:
+            sp.sp_image_free(im)
+        return ph
+
+    per_it = {}
+    for name, alg, nbytes in (("dm", dm, 211), ("er", er, 102)):
+        ph = phaser(alg, None)
+        ms = C.c_float()
+        if sp.spx_phaser_time_iterations(ph, 5, C.byref(ms)) != 0 or \
+                sp.spx_phaser_time_iterations(ph, 40, C.byref(ms)) != 0:
+            raise RuntimeError(sp.last_error())
+        t = ms.value / 1e3 / 40
+        per_it[name] = {"us_per_iteration": t * 1e6, "algorithmic_bytes": nbytes * N,
+                        "achieved_GBps": nbytes * N / t / 1e9, "frac": nbytes * N / t / PEAK_HBM}
+        sp.sp_phaser_free(ph)
+    ph = phaser(dm, sarr)
+    sp.sp_phaser_efourier(ph, None)
+    t0 = time.perf_counter()
+    rc = sp.sp_phaser_iterate(ph, dm_its)
+    if rc == 0:
+        rc = sp.sp_phaser_set_phasing_algorithm(ph, er)
+    if rc == 0:
+        rc = sp.sp_phaser_iterate(ph, er_its)
+    fc = C.c_float()
+    ef = sp.sp_phaser_efourier(ph, C.byref(fc))
+    t_rec = time.perf_counter() - t0
+    frac = sp.sp_phaser_support_fraction(ph)
+    sp.sp_phaser_free(ph)
+    return {"workload": f"256^3 ellipsoid pattern (200 Gaussians, 1e8 photons, beamstop 5): DM beta 0.8 "
+                        f"(Elser gammas) {dm_its} iterations then ER {er_its}, area shrinkwrap every {period}",
+            "rc": rc, "seconds": t_rec, "iter_per_s": total / t_rec, "final_efourier": ef, "fcfo": fc.value,
+            "support_fraction": frac, "plain_iterations": per_it}
 
 
 def _serving_pattern(args):
@@ -357,6 +421,7 @@ def main():
     t_iter, roofline, it_roof, recon = gpu_bench(args)
     batched = None if args.no_batched else batched_bench()
     serving = None if args.no_serving else serving_bench()
+    volume = None if args.no_volume else volume_bench()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
@@ -378,6 +443,7 @@ def main():
         "reconstruction_1024": recon,
         "batched": batched,
         "serving": serving,
+        "volume_256": volume,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

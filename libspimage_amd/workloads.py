@@ -28,6 +28,17 @@ class Pattern:
     seed: int
 
 
+def _fftn(x, inverse=False):
+    """numpy's FFT; volumes above 4 Mi elements use scipy's multi-threaded one."""
+    if x.size > (1 << 22):
+        import os
+
+        import scipy.fft
+        f = scipy.fft.ifftn if inverse else scipy.fft.fftn
+        return f(x, workers=min(16, os.cpu_count() or 1))
+    return np.fft.ifftn(x) if inverse else np.fft.fftn(x)
+
+
 def _grid(shape):
     axes = [np.arange(n) - n // 2 for n in shape]
     return np.meshgrid(*axes, indexing="ij")
@@ -46,13 +57,14 @@ def corner_distance(shape) -> np.ndarray:
 
 
 def make_pattern(shape, n_gauss=20, sigma=(2.0, 6.0), radius=40.0, photons=1e5, beamstop=4.0,
-                 dilate=3.0, seed=0, complex_blobs=0, irregular=False) -> Pattern:
+                 dilate=3.0, seed=0, complex_blobs=0, irregular=False, window=None) -> Pattern:
     """Build one seeded pattern.
 
     `radius` is a scalar (disc / sphere) or a tuple of semi-axes (ellipse /
     ellipsoid) in pixels.  `complex_blobs` adds that many uniform-random-phase
     blobs.  `irregular` replaces the disc by a low-pass random field
-    thresholded around it."""
+    thresholded around it.  `window` (in units of the blob's sigma) limits
+    each Gaussian to a box around its centre, for large volumes."""
     shape = tuple(int(s) for s in shape)
     rng = np.random.default_rng(seed)
     g = _grid(shape)
@@ -63,7 +75,7 @@ def make_pattern(shape, n_gauss=20, sigma=(2.0, 6.0), radius=40.0, photons=1e5, 
         noise = rng.standard_normal(shape)
         freqs = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
         kk = np.sqrt(sum(f * f for f in freqs))
-        field = np.real(np.fft.ifftn(np.fft.fftn(noise) * np.exp(-(kk / 0.01) ** 2)))
+        field = np.real(_fftn(_fftn(noise) * np.exp(-(kk / 0.01) ** 2), inverse=True))
         field /= np.abs(field).max() + 1e-30
         region = (rr + 0.35 * field) <= 1.0
     obj = np.zeros(shape, dtype=np.complex128)
@@ -76,15 +88,20 @@ def make_pattern(shape, n_gauss=20, sigma=(2.0, 6.0), radius=40.0, photons=1e5, 
                 break
         s = rng.uniform(*sigma)
         a = rng.uniform(0.5, 1.0)
-        r2 = sum((gi - ci) ** 2 for gi, ci in zip(g, c))
+        if window is None:
+            box = tuple(slice(None) for _ in shape)
+        else:
+            box = tuple(slice(max(0, int(np.floor(ci - window * s)) + n // 2),
+                              min(n, int(np.ceil(ci + window * s)) + n // 2 + 1)) for ci, n in zip(c, shape))
+        r2 = sum((gi[box] - ci) ** 2 for gi, ci in zip(g, c))
         blob = a * np.exp(-r2 / (2 * s * s))
         if b >= n_gauss:
             blob = blob * np.exp(2j * np.pi * rng.uniform())
-        obj += blob
+        obj[box] += blob
     obj[~region] = 0
     obj = np.fft.ifftshift(obj)
     region = np.fft.ifftshift(region)
-    intens = np.abs(np.fft.fftn(obj)) ** 2
+    intens = np.abs(_fftn(obj)) ** 2
     intens *= photons / intens.sum()
     counts = rng.poisson(intens).astype(np.float64)
     amp = np.sqrt(counts).astype(np.float32)
@@ -93,12 +110,12 @@ def make_pattern(shape, n_gauss=20, sigma=(2.0, 6.0), radius=40.0, photons=1e5, 
     # support: region dilated by `dilate` pixels (distance transform by FFT
     # convolution with a ball)
     ball = np.fft.ifftshift(np.sqrt(sum(gi.astype(np.float64) ** 2 for gi in g)) <= dilate)
-    conv = np.real(np.fft.ifftn(np.fft.fftn(region.astype(np.float64)) * np.fft.fftn(ball)))
+    conv = np.real(_fftn(_fftn(region.astype(np.float64)) * _fftn(ball), inverse=True))
     support = (conv > 0.5).astype(np.float32)
     # random-phase start: the reference's SpModelRandomPhases recipe
     # (src/phasing.c:626-646) with this generator
     phase = rng.uniform(0, 2 * np.pi, shape)
-    start = (np.fft.ifftn(amp * np.exp(1j * phase))).astype(np.complex64)
+    start = (_fftn(amp * np.exp(1j * phase), inverse=True)).astype(np.complex64)
     return Pattern(amp, mask, support, obj.astype(np.complex64), start, seed)
 
 
@@ -118,10 +135,11 @@ def blob_1024(seed=1000) -> Pattern:
 
 
 def ellipsoid_256(seed=3000) -> Pattern:
-    """3-D reconstruction, 256^3: 200 Gaussians (sigma 2-8 voxels) in an
-    ellipsoid with semi-axes 60/50/40, 1e8 photons, spherical beamstop 5."""
+    """3-D reconstruction, 256^3: 200 Gaussians (sigma 2-8 voxels, evaluated
+    within 5 sigma) in an ellipsoid with semi-axes 60/50/40, 1e8 photons,
+    spherical beamstop 5."""
     return make_pattern((256, 256, 256), n_gauss=200, sigma=(2, 8), radius=(40, 50, 60), photons=1e8,
-                        beamstop=5, dilate=3, seed=seed)
+                        beamstop=5, dilate=3, seed=seed, window=5.0)
 
 
 def disc_scaled(n: int, seed: int) -> Pattern:
