@@ -155,7 +155,7 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     (blur 3.0 -> 0.7 px over the run, 8 % of the pixels).
 
     Batched: one SpxBatch per size class advances all its reconstructions
-    together (HIP-graph replay between support updates).  Plain: the
+    together (one launch per pass for the whole class).  Plain: the
     reference-style loop, one SpPhaser per reconstruction, timed on
     `plain_per_size` reconstructions per size class from the same starts and
     extrapolated to the size class's count.  Inputs are uploaded before the
@@ -185,7 +185,7 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
         b.init_model(sp.SpModelRandomPhases, seed + n)
         b.set_algorithm(hio)
         b.set_support_schedule(sarr)
-        b.use_graphs(True)
+        b.use_graphs(False)
         starts = [b.model(r) for r in range(plain_per_size)]
         b.synchronize()
         t0 = time.perf_counter()

@@ -37,8 +37,8 @@ bool device_available() {
 
 // ---------------------------------------------------------------------------
 // twiddle tables: for stage s >= 1 of plan (L, E), entry
-// twoff[s] + j*tw_stride(R) + (k-1) holds exp(-2 pi i j k / (Ns R)),
-// evaluated in double and rounded to float (rows padded to an even length)
+// twoff[s] + (k-1)*Ns + j holds exp(-2 pi i j k / (Ns R)), evaluated in
+// double and rounded to float
 // ---------------------------------------------------------------------------
 const float2* twiddles(int L, int E) {
   static std::mutex mu;
@@ -54,7 +54,7 @@ const float2* twiddles(int L, int E) {
     for (int j = 0; j < Ns; ++j)
       for (int k = 1; k < R; ++k) {
         const double a = -2.0 * M_PI * (double)j * (double)k / (double)(Ns * R);
-        h[p.twoff[s] + j * tw_stride(R) + (k - 1)] = make_float2((float)cos(a), (float)sin(a));
+        h[p.twoff[s] + (k - 1) * Ns + j] = make_float2((float)cos(a), (float)sin(a));
       }
   }
   float2* d = nullptr;
@@ -98,9 +98,12 @@ static float time_launches(hipStream_t s, F&& launch) {
   return (hipGetLastError() == hipSuccess) ? best : 1e30f;
 }
 
+static std::mutex tune_mu;
+static std::map<LengthEntry*, int> tune_done;  // bit 1 row tuned, bit 2 column tuned
+
 void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
-  static std::mutex mu;
-  static std::map<LengthEntry*, int> done;  // bit 1 row tuned, bit 2 column tuned
+  std::mutex& mu = tune_mu;
+  std::map<LengthEntry*, int>& done = tune_done;
   std::lock_guard<std::mutex> lock(mu);
   const int bit = col_geo ? 2 : 1;
   if (done[e] & bit) return;
@@ -167,6 +170,88 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
   }
   if (col_geo) e->col_choice = choice;
   else e->row_choice = choice;
+}
+
+static void tune_pair(LengthEntry* ex, LengthEntry* ey, int lines, const LineGeom& gy) {
+  {
+    std::lock_guard<std::mutex> lock(tune_mu);
+    const bool need_row = !(tune_done[ex] & 1), need_col = !(tune_done[ey] & 2);
+    const char* force = getenv("SPX_FFT_E");
+    const char* notune = getenv("SPX_NO_TUNE");
+    const bool skip = (force && atoi(force) > 0) || (notune && atoi(notune) == 1) || (ex->nvar < 2 && ey->nvar < 2);
+    if (!need_row && !need_col) return;
+    if (!skip) {
+      tune_done[ex] |= 1;
+      tune_done[ey] |= 2;
+      // both passes in sequence, as one iteration runs them: a variant's
+      // cost depends on where the previous pass left the data
+      const long L = ex->L;
+      const int tl = (long)lines * L > (1L << 24) ? (int)((1L << 24) / L) : lines;  // cap the scratch
+      LineGeom g = gy;
+      if ((long)g.nouter * g.outer_stride > (long)tl * L) g.nouter = (int)(((long)tl * L) / g.outer_stride);
+      if (g.nouter < 1) g.nouter = 1;
+      const int nl = (int)(((long)g.nouter * g.outer_stride) / L);
+      const long n = (long)nl * L;
+      hipStream_t s = nullptr;
+      float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+      float* amp = nullptr;
+      uint8_t* fl = nullptr;
+      bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+                hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
+                hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+                hipMalloc(&fl, (size_t)n) == hipSuccess;
+      ok = ok && hipMemsetAsync(w, 0, sizeof(float2) * n, s) == hipSuccess &&
+           hipMemsetAsync(g0, 0, sizeof(float2) * n, s) == hipSuccess &&
+           hipMemsetAsync(amp, 0, sizeof(float) * n, s) == hipSuccess &&
+           hipMemsetAsync(fl, 0, (size_t)n, s) == hipSuccess;
+      if (ok) {
+        float best = 1e30f;
+        int br = ex->row_choice, bc = ey->col_choice;
+        for (int r = 0; r < ex->nvar; ++r) {
+          if (!need_row && r != ex->row_choice) continue;
+          for (int c = 0; c < ey->nvar; ++c) {
+            if (!need_col && c != ey->col_choice) continue;
+            ColArgs ca{};
+            ca.in = w;
+            ca.W = w;
+            ca.geo = g;
+            ca.ampx = amp;
+            UpdateArgs u{};
+            u.W = w;
+            u.gcur = g0;
+            u.gnew = g1;
+            u.flags = fl;
+            u.nlines = nl;
+            u.lines_per_rec = nl;
+            u.alg = kER;
+            u.scale = 1.f;
+            u.nsize = 1.f;
+            const Variant &vr = ex->var[r], &vc = ey->var[c];
+            const float ms = time_launches(s, [&] {
+              vc.col(kColProject, ca, s);
+              vr.update(u, false, s);
+            });
+            if (ms < best) {
+              best = ms;
+              br = r;
+              bc = c;
+            }
+          }
+        }
+        ex->row_choice = br;
+        ey->col_choice = bc;
+      }
+      (void)hipStreamSynchronize(s);
+      void* bufs[] = {w, g0, g1, amp, fl};
+      for (void* p : bufs)
+        if (p) (void)hipFree(p);
+      if (s) (void)hipStreamDestroy(s);
+      return;
+    }
+  }
+  // forced or disabled tuning: the per-length rule
+  tune_length(ex, lines, nullptr);
+  tune_length(ey, 0, &gy);
 }
 
 // ---------------------------------------------------------------------------
@@ -472,9 +557,13 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   // blocks of nx*ny adjacent lines
   sh->gy = LineGeom{nx, nz * nrec, (long)nx * ny, (long)nx};
   sh->gz = LineGeom{nx * ny, nrec, (long)nx * ny * nz, (long)nx * ny};
-  if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
-  if (found[1]) tune_length(found[1], 0, &sh->gy);
-  if (found[2]) tune_length(found[2], 0, &sh->gz);
+  if (found[0] && found[1] && !found[2]) {
+    tune_pair(found[0], found[1], ny * nrec, sh->gy);
+  } else {
+    if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
+    if (found[1]) tune_length(found[1], 0, &sh->gy);
+    if (found[2]) tune_length(found[2], 0, &sh->gz);
+  }
   return 0;
 }
 
@@ -916,14 +1005,8 @@ int Engine::time_iterations(const IterParams& p, const float* betas, int n, floa
   if (int r = ensure_R()) return r;
   if (p.alg == kDiffMap)
     if (int r = ensure_Rf(p.gamma1)) return r;
-  bool constant = n >= 2 && !alg_rec && !beta_rec;
-  for (int i = 1; i < n && constant; ++i) constant = betas[i] == betas[0];
-  if (constant) {
-    // capture outside the timed region
-    if (int r = iterate_graph(p, betas[0], 0)) return r;
-  }
   SPX_HIP(hipEventRecord(ev0, s));
-  if (int r = constant ? iterate_graph(p, betas[0], n) : iterate(p, betas, n, false)) return r;
+  if (int r = iterate(p, betas, n, false)) return r;
   SPX_HIP(hipEventRecord(ev1, s));
   SPX_HIP(hipEventSynchronize(ev1));
   SPX_HIP(hipEventElapsedTime(ms, ev0, ev1));

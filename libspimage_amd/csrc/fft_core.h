@@ -37,10 +37,6 @@ struct FftPlan {
   bool ok;        // L factors into the supported radices
 };
 
-// twiddle row stride of a radix-R stage: R-1 entries padded to an even count
-// so a row can be read as float4 pairs
-constexpr int tw_stride(int R) { return ((R - 1) + 1) & ~1; }
-
 constexpr int ilog2c(int v) {
   int e = 0;
   while ((1 << e) < v) ++e;
@@ -105,7 +101,7 @@ constexpr FftPlan make_plan(int L, int E) {
   for (int s = 0; s < p.n; ++s) {
     p.ns[s] = ns;
     p.twoff[s] = off;
-    if (s > 0) off += ns * tw_stride(p.r[s]);
+    if (s > 0) off += ns * (p.r[s] - 1);
     ns *= p.r[s];
     if (p.r[s] > mr) mr = p.r[s];
   }
@@ -128,13 +124,31 @@ struct PlanOf {
   static constexpr int B0 = (L / E0) / T;  // stage-0 butterflies per thread
   static constexpr int BL = (L / EL) / T;  // last-stage butterflies per thread
   static constexpr bool PAL = (E0 == EL);  // output positions == input positions
-  static constexpr int TW = P.twtotal > 0 ? P.twtotal : 2;  // twiddle entries (even)
   static_assert(B0 * E0 == NREG && BL * EL == NREG, "edge stages must divide evenly over the threads");
+  // butterflies of stage s one thread computes
+  static constexpr int bpt(int s) { return ((L / P.r[s]) + T - 1) / T; }
+  // first twiddle register of stage s >= 1: stages 1..s-1 each hold
+  // bpt * (R - 1) values
+  static constexpr int twreg(int s) {
+    int o = 0;
+    for (int q = 1; q < s; ++q) o += bpt(q) * (P.r[q] - 1);
+    return o;
+  }
+  static constexpr int NTW = twreg(P.n);  // twiddle registers per thread
 };
 
 // the edge registers of one thread
 template <int L, int E>
 using RegLine = float2[PlanOf<L, E>::NREG];
+
+// The twiddles one thread applies, for every stage after the first, read
+// from the global table once at kernel start (the table is a few KiB and
+// shared by every work-group, so these reads hit in the caches) and used by
+// both the inverse and the forward transform of a pass.
+template <int L, int E>
+struct TwRegs {
+  float2 w[PlanOf<L, E>::NTW > 0 ? PlanOf<L, E>::NTW : 1];
+};
 
 // ---------------------------------------------------------------------------
 // Complex helpers on float2 (x = re, y = im).
@@ -280,27 +294,40 @@ struct LdsView {
   }
 };
 
-// Twiddles of butterfly j of a stage (forward sense), R-1 values.
-template <int R>
-__device__ __forceinline__ void load_twiddles(const float2* __restrict__ row, float2 (&w)[R]) {
-  constexpr int S = tw_stride(R);
-  static_assert(S % 2 == 0, "twiddle rows are float4 aligned");
-  const float4* r4 = reinterpret_cast<const float4*>(row);
+// Global twiddle table of plan (L, E): stage s >= 1 keeps exp(-2 pi i j k /
+// (Ns R)) at twoff[s] + (k-1) Ns + j, so lanes with consecutive butterflies
+// read consecutive entries.
+template <int L, int E, int S>
+__device__ __forceinline__ void load_tw_stage(const float2* __restrict__ tw, int t, TwRegs<L, E>& r) {
+  constexpr FftPlan P = PlanOf<L, E>::P;
+  if constexpr (S < P.n) {
+    constexpr int R = P.r[S], NS = P.ns[S], NB = L / R, T = P.T;
+    constexpr int BPT = PlanOf<L, E>::bpt(S), O = PlanOf<L, E>::twreg(S);
 #pragma unroll
-  for (int q = 0; q < S / 2; ++q) {
-    const float4 v = r4[q];
-    if (2 * q + 1 < R) w[2 * q + 1] = make_float2(v.x, v.y);
-    if (2 * q + 2 < R) w[2 * q + 2] = make_float2(v.z, v.w);
+    for (int i = 0; i < BPT; ++i) {
+      const int b = t + i * T;
+      if (BPT * T == NB || b < NB) {
+        const int j = b % NS;
+#pragma unroll
+        for (int k = 1; k < R; ++k) r.w[O + i * (R - 1) + (k - 1)] = tw[P.twoff[S] + (k - 1) * NS + j];
+      }
+    }
+    load_tw_stage<L, E, S + 1>(tw, t, r);
   }
+}
+
+template <int L, int E>
+__device__ __forceinline__ void load_tw(const float2* __restrict__ tw, int t, TwRegs<L, E>& r) {
+  load_tw_stage<L, E, 1>(tw, t, r);
 }
 
 // Middle stage S: LDS `in` -> LDS `out` (no barrier inside).
 template <int L, int E, int S, int DIR, class View>
-__device__ __forceinline__ void mid_stage(const View& in, const View& out, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void mid_stage(const View& in, const View& out, int t, const TwRegs<L, E>& tr) {
   constexpr FftPlan P = PlanOf<L, E>::P;
   constexpr int R = P.r[S], NS = P.ns[S], T = P.T;
   constexpr int NB = L / R;
-  constexpr int BPT = (NB + T - 1) / T;
+  constexpr int BPT = PlanOf<L, E>::bpt(S), O = PlanOf<L, E>::twreg(S);
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int b = t + i * T;
@@ -309,10 +336,8 @@ __device__ __forceinline__ void mid_stage(const View& in, const View& out, int t
 #pragma unroll
       for (int k = 0; k < R; ++k) x[k] = in.at(b + k * NB);
       const int j = b % NS;
-      float2 w[R];
-      load_twiddles<R>(tw + P.twoff[S] + j * tw_stride(R), w);
 #pragma unroll
-      for (int k = 1; k < R; ++k) x[k] = twmul<DIR>(x[k], w[k]);
+      for (int k = 1; k < R; ++k) x[k] = twmul<DIR>(x[k], tr.w[O + i * (R - 1) + (k - 1)]);
       dft<R, DIR>(x);
       const int base = (b - j) * R + j;
 #pragma unroll
@@ -322,14 +347,14 @@ __device__ __forceinline__ void mid_stage(const View& in, const View& out, int t
 }
 
 template <int L, int E, int S, int DIR, class View>
-__device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, const TwRegs<L, E>& tr) {
   constexpr FftPlan P = PlanOf<L, E>::P;
   if constexpr (S < P.n - 1) {
     // stage S reads the buffer stage S-1 wrote: a for odd S, b for even S
-    if constexpr (S % 2 == 1) mid_stage<L, E, S, DIR>(a, b, t, tw);
-    else mid_stage<L, E, S, DIR>(b, a, t, tw);
+    if constexpr (S % 2 == 1) mid_stage<L, E, S, DIR>(a, b, t, tr);
+    else mid_stage<L, E, S, DIR>(b, a, t, tr);
     __syncthreads();
-    mid_stages<L, E, S + 1, DIR>(a, b, t, tw);
+    mid_stages<L, E, S + 1, DIR>(a, b, t, tr);
   }
 }
 
@@ -340,7 +365,7 @@ __device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, 
 // LDS next must synchronise first.
 template <int L, int E, int DIR, class View>
 __device__ __forceinline__ void fft_line(RegLine<L, E>& v, const View& a, const View& b, int t,
-                                         const float2* __restrict__ tw) {
+                                         const TwRegs<L, E>& tr) {
   constexpr FftPlan P = PlanOf<L, E>::P;
   constexpr int T = P.T;
   if constexpr (P.n == 1) {
@@ -356,7 +381,7 @@ __device__ __forceinline__ void fft_line(RegLine<L, E>& v, const View& a, const 
       for (int r = 0; r < E0; ++r) a.at(bb * E0 + r) = v[i * E0 + r];
     }
     __syncthreads();
-    mid_stages<L, E, 1, DIR>(a, b, t, tw);
+    mid_stages<L, E, 1, DIR>(a, b, t, tr);
     // last stage reads the buffer written by stage n-2
     constexpr int SL = P.n - 1;
     constexpr int EL = P.r[SL], NS = P.ns[SL], BL = PlanOf<L, E>::BL;
@@ -367,10 +392,9 @@ __device__ __forceinline__ void fft_line(RegLine<L, E>& v, const View& a, const 
       float2* x = v + i * EL;
 #pragma unroll
       for (int k = 0; k < EL; ++k) x[k] = src.at(bb + k * NS);
-      float2 w[EL];
-      load_twiddles<EL>(tw + P.twoff[SL] + bb * tw_stride(EL), w);
+      constexpr int O = PlanOf<L, E>::twreg(SL);
 #pragma unroll
-      for (int k = 1; k < EL; ++k) x[k] = twmul<DIR>(x[k], w[k]);
+      for (int k = 1; k < EL; ++k) x[k] = twmul<DIR>(x[k], tr.w[O + i * (EL - 1) + (k - 1)]);
       dft<EL, DIR>(x);
     }
   }

@@ -205,15 +205,6 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV
 }
 
 
-// Copies the length's twiddle table into LDS (issued before the data loads
-// complete; the first barrier inside fft_line orders it before first use).
-template <int L, int E, int NT>
-__device__ __forceinline__ void stage_twiddles(float4* lds, const float2* __restrict__ tw) {
-  constexpr int N4 = PlanOf<L, E>::TW / 2;
-  const float4* g = reinterpret_cast<const float4*>(tw);
-  for (int i = threadIdx.x; i < N4; i += NT) lds[i] = g[i];
-}
-
 // ---------------------------------------------------------------------------
 // Row kernels: M lines along x per work-group, T threads per line.
 // ---------------------------------------------------------------------------
@@ -232,7 +223,6 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
-  __shared__ float4 tws[PL::TW / 2];
   __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
@@ -240,26 +230,25 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
   const int line = blk * M + g;
   const bool act = line < a.nlines;
   const View va{shA, g}, vb{shB, g};
-  const long base = (long)line * L;
+  // idle lanes of a partial last work-group address the last line (loads
+  // stay in bounds and need no predicate); only their stores are skipped
+  const long base = (long)(act ? line : a.nlines - 1) * L;
 
   float2 v[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    float2 x = make_float2(0.f, 0.f);
-    if (act) {
-      x = a.in[base + in_pos<L, E>(t, q)];
-      if constexpr (MODE == kRowAbsFwd) x = make_float2(cabsf_ref(x), 0.f);
-    }
+    float2 x = a.in[base + in_pos<L, E>(t, q)];
+    if constexpr (MODE == kRowAbsFwd) x = make_float2(cabsf_ref(x), 0.f);
     v[q] = x;
   }
-  stage_twiddles<L, E, NT>(tws, a.tw);
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
   if constexpr (MODE == kRowBlurOut) {
     if (a.hist)
       for (int i = tid; i < 2048; i += NT) hsh[i] = 0u;
   }
-  const float2* twl = reinterpret_cast<const float2*>(tws);
-  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, E, +1>(v, va, vb, t, twl);
-  else fft_line<L, E, -1>(v, va, vb, t, twl);
+  if constexpr (MODE == kRowInv || MODE == kRowBlurOut) fft_line<L, E, +1>(v, va, vb, t, twr);
+  else fft_line<L, E, -1>(v, va, vb, t, twr);
 
   if constexpr (MODE == kRowBlurOut) {
     // the work-group's lines belong to one reconstruction unless lines of
@@ -319,39 +308,46 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
-  __shared__ float4 tws[PL::TW / 2];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
   const bool act = line < a.nlines;
   const View va{shA, g}, vb{shB, g};
-  const long base = (long)line * L;
+  // idle lanes of a partial last work-group address the last line (loads
+  // stay in bounds and need no predicate); only their stores are skipped
+  const long base = (long)(act ? line : a.nlines - 1) * L;
 
   float2 v[NR], vf[DM ? NR : 1], g0v[NR];
   uint8_t fl[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    v[q] = act ? a.W[base + in_pos<L, E>(t, q)] : make_float2(0.f, 0.f);
-    if constexpr (DM) vf[q] = act ? a.Wf[base + in_pos<L, E>(t, q)] : make_float2(0.f, 0.f);
+    v[q] = a.W[base + in_pos<L, E>(t, q)];
+    if constexpr (DM) vf[q] = a.Wf[base + in_pos<L, E>(t, q)];
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     const long idx = base + out_pos<L, E>(t, q);
-    g0v[q] = act ? a.gcur[idx] : make_float2(0.f, 0.f);
-    fl[q] = act ? a.flags[idx] : (uint8_t)0;
+    g0v[q] = a.gcur[idx];
+    fl[q] = a.flags[idx];
   }
-  stage_twiddles<L, E, NT>(tws, a.tw);
-  const float2* twl = reinterpret_cast<const float2*>(tws);
-  fft_line<L, E, +1>(v, va, vb, t, twl);
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  fft_line<L, E, +1>(v, va, vb, t, twr);
   if constexpr (DM) {
     __syncthreads();
-    fft_line<L, E, +1>(vf, va, vb, t, twl);
+    fft_line<L, E, +1>(vf, va, vb, t, twr);
   }
 
   const int rec = act ? line / a.lines_per_rec : 0;
   const float beta = a.beta_rec ? a.beta_rec[rec] : a.beta;
   const int alg = a.alg_rec ? a.alg_rec[rec] : a.alg;
+  // outside the support g1 = cp * gp + cg * g0: HIO g0 - beta gp
+  // (src/phasing.c:1160-1168), RAAR beta g0 + (1 - 2 beta) gp
+  // (src/phasing.c:1194-1215), ER 0 (src/phasing.c:1128-1136); inside, gp
+  const bool er = !(alg == kHIO || alg == kRAAR);
+  const float cp = alg == kHIO ? -beta : 1.f - 2.f * beta;
+  const float cg = alg == kHIO ? 1.f : beta;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (!act) continue;
@@ -363,16 +359,10 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
     if constexpr (!DM) {
       const float2 gp = make_float2(P.x * a.scale, P.y * a.scale);
       if (a.gp) a.gp[idx] = gp;
-      if (in) {
-        g1 = gp;
-      } else if (alg == kHIO) {
-        g1 = make_float2(g0.x - gp.x * beta, g0.y - gp.y * beta);
-      } else if (alg == kRAAR) {
-        const float c1 = 1.f - 2.f * beta;
-        g1 = make_float2(gp.x * c1 + g0.x * beta, gp.y * c1 + g0.y * beta);
-      } else {
-        g1 = make_float2(0.f, 0.f);
-      }
+      float2 o = make_float2(fmaf(cp, gp.x, cg * g0.x), fmaf(cp, gp.y, cg * g0.y));
+      if (er) o = make_float2(0.f, 0.f);
+      const float2 gi = apply_constraints(gp, a.constraints);
+      g1 = make_float2(in ? gi.x : o.x, in ? gi.y : o.y);
     } else {
       // src/phasing.c:1243-1252; gp stays unscaled like the CPU engine
       if (a.gp) a.gp[idx] = P;
@@ -386,7 +376,9 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
       g1.x -= beta * Pf.x / a.nsize;
       g1.y -= beta * Pf.y / a.nsize;
     }
-    if (in) g1 = apply_constraints(g1, a.constraints);
+    if constexpr (DM) {
+      if (in) g1 = apply_constraints(g1, a.constraints);
+    }
     a.gnew[idx] = g1;
     v[q] = g1;
     // f1 of the next iteration (src/phasing.c:1260-1274)
@@ -395,10 +387,10 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
   out_to_in<L, E>(v, va, t);
   if constexpr (DM) out_to_in<L, E>(vf, va, t);
   __syncthreads();
-  fft_line<L, E, -1>(v, va, vb, t, twl);
+  fft_line<L, E, -1>(v, va, vb, t, twr);
   if constexpr (DM) {
     __syncthreads();
-    fft_line<L, E, -1>(vf, va, vb, t, twl);
+    fft_line<L, E, -1>(vf, va, vb, t, twr);
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
@@ -434,7 +426,6 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
   using View = LdsView<L, C, true>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
-  __shared__ float4 tws[PL::TW / 2];
   __shared__ double red[(MODE == kColEFourier) ? 4 * 32 : 1];
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
@@ -443,25 +434,26 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
   const int outer = blk / ngrp;
   const int x = (blk - outer * ngrp) * C + c;
   const bool act = x < a.geo.ncontig;
-  const long base = (long)outer * a.geo.outer_stride + x;
+  // idle lanes address the last line of the group (in bounds, no predicate)
+  const long base = (long)outer * a.geo.outer_stride + (act ? x : a.geo.ncontig - 1);
   const long st = a.geo.axis_stride;
   const View va{shA, c}, vb{shB, c};
 
   float2 v[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) v[q] = act ? a.in[base + in_pos<L, E>(t, q) * st] : make_float2(0.f, 0.f);
+  for (int q = 0; q < NR; ++q) v[q] = a.in[base + in_pos<L, E>(t, q) * st];
   // amplitudes of the projection, fetched with the data
   float ampv[(MODE == kColProject || MODE == kColEFourier) ? NR : 1];
   if constexpr (MODE == kColProject || MODE == kColEFourier) {
     if (MODE == kColEFourier || a.objective == 0) {
 #pragma unroll
-      for (int q = 0; q < NR; ++q) ampv[q] = act ? a.ampx[base + out_pos<L, E>(t, q) * st] : 0.f;
+      for (int q = 0; q < NR; ++q) ampv[q] = a.ampx[base + out_pos<L, E>(t, q) * st];
     }
   }
-  stage_twiddles<L, E, NT>(tws, a.tw);
-  const float2* twl = reinterpret_cast<const float2*>(tws);
-  if constexpr (MODE == kColInv) fft_line<L, E, +1>(v, va, vb, t, twl);
-  else fft_line<L, E, -1>(v, va, vb, t, twl);
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  if constexpr (MODE == kColInv) fft_line<L, E, +1>(v, va, vb, t, twr);
+  else fft_line<L, E, -1>(v, va, vb, t, twr);
 
   if constexpr (MODE == kColProject) {
 #pragma unroll
@@ -479,7 +471,7 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
     }
     out_to_in<L, E>(v, va, t);
     __syncthreads();
-    fft_line<L, E, +1>(v, va, vb, t, twl);
+    fft_line<L, E, +1>(v, va, vb, t, twr);
   } else if constexpr (MODE == kColBlur) {
     // Gaussian transfer function of sp_gaussian_blur (src/image_filter.c:80-91)
     // with dist_to_top_left (src/image_util.c:1012-1032), computed in double
@@ -505,7 +497,7 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
     }
     out_to_in<L, E>(v, va, t);
     __syncthreads();
-    fft_line<L, E, +1>(v, va, vb, t, twl);
+    fft_line<L, E, +1>(v, va, vb, t, twr);
   } else if constexpr (MODE == kColEFourier) {
     // src/phasing.c:879-888: measured pixels add (|F|-|amp|)^2 and amp^2,
     // unmeasured ones |F|^2; FcFo sums |amp| and |F| over measured pixels.

@@ -14,7 +14,7 @@ Scheduling:
   shapes never mix, so no reconstruction is padded or resized and each
   result equals the same request run alone through sp_phaser_iterate;
 * continuous batching -- a batch advances in quanta of `quantum`
-  iterations (HIP-graph replays).  Between quanta finished reconstructions
+  iterations.  Between quanta finished reconstructions
   leave and waiting requests of the class take their slots; when every slot
   of a class is busy and requests wait, a further batch is created for them;
 * admission by free device memory -- a batch is created only if its
@@ -139,7 +139,7 @@ class Server:
             return None
         b = sp.Batch(shape, cap)
         b.set_algorithm(self._base)
-        b.use_graphs(True)
+        b.use_graphs(False)
         bt = _Batch(b, [None] * cap, algs=np.full(cap, sp.SpER, np.int32), betas=np.zeros(cap, np.float32))
         self.classes.setdefault(shape, []).append(bt)
         self.batches_created += 1

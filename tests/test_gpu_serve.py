@@ -44,7 +44,9 @@ def test_served_equals_alone(sp, realtime):
     srv = serve.Server(quantum=50, max_pixels_per_batch=2 * 64 * 64, reserve_bytes=0, keep_models=True)
     res = srv.run(reqs, realtime=realtime)
     assert sorted(r.rid for r in res) == list(range(len(reqs)))
-    assert srv.batches_created >= 4
+    if not realtime:
+        # all requests queued at once: the 64x64 class alone needs 3 batches
+        assert srv.batches_created >= 4
     srv.close()
     plain = serve.PlainRunner()
     by_id = {r.rid: r for r in res}
