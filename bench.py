@@ -94,12 +94,18 @@ def gpu_bench(args):
     npass = max(200, args.steps // 4)
     if sp.spx_phaser_time_passes(ph, npass, C.byref(mf), C.byref(mr)) != 0:
         raise RuntimeError(sp.last_error())
-    t_f, t_r = mf.value / 1e3 / npass, mr.value / 1e3 / npass
+    t_f_ev, t_r_ev = mf.value / 1e3 / npass, mr.value / 1e3 / npass
+    # each HIP event between passes adds a dispatch gap: the difference
+    # between the event-bracketed and the plain iteration time, split evenly
+    # over the two passes, is taken off each pass
+    gap = max(0.0, (t_f_ev + t_r_ev - t_iter) / 2)
+    t_f, t_r = t_f_ev - gap, t_r_ev - gap
+    tiled = sp.spx_phaser_tiled(ph) == 1
     sp.sp_phaser_free(ph)
 
     passes = {
-        "real_space_pass": {"bytes_per_pixel": BYTES_REAL_PASS, "seconds": t_r},
-        "fourier_space_pass": {"bytes_per_pixel": BYTES_FOURIER_PASS, "seconds": t_f},
+        "real_space_pass": {"bytes_per_pixel": BYTES_REAL_PASS, "seconds": t_r, "event_seconds": t_r_ev},
+        "fourier_space_pass": {"bytes_per_pixel": BYTES_FOURIER_PASS, "seconds": t_f, "event_seconds": t_f_ev},
     }
     dom_name = max(passes, key=lambda k: passes[k]["seconds"])
     dom = passes[dom_name]
@@ -122,7 +128,9 @@ def gpu_bench(args):
         "achieved": it_achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": it_achieved / PEAK_HBM,
         "algorithmic_bytes": BYTES_HIO * N,
         "passes_us": {k: v["seconds"] * 1e6 for k, v in passes.items()},
-        "fft_elements_per_thread": dict(zip(("row", "column"), sp.fft_variant(1024))),
+        "passes_event_bracketed_us": {k: v["event_seconds"] * 1e6 for k, v in passes.items()},
+        "fft_elements_per_thread": dict(zip(("row", "column"), sp.fft_variant(1024, tiled))),
+        "fourier_space_layout": "4-column tiled" if tiled else "row-major",
     }
 
     # the 1024^2 reconstruction: RAAR beta 0.87, 300 iterations, area

@@ -31,8 +31,16 @@ spimage_EXPORT const char *spx_last_error(void);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
 /* Elements per thread (4, 8 or 16) of the transform variant the tuner chose
- * for lines of length L in row passes and in column passes. */
-spimage_EXPORT int spx_fft_variant(int L, int *row_elements, int *col_elements);
+ * for lines of length L in row passes and in column passes, for 2-D shapes
+ * whose Fourier-space arrays are row-major (tiled = 0) or in the 4-column
+ * tiled layout (tiled = 1). */
+spimage_EXPORT int spx_fft_variant(int L, int tiled, int *row_elements, int *col_elements);
+
+/* 1 when the phaser's engine keeps its Fourier-space work arrays (the
+ * row-transformed iterate, amplitudes, margins, phased amplitudes) in the
+ * 4-column tiled layout, 0 when row-major, negative without an engine.  The
+ * iterates, supports and every array crossing this ABI stay row-major. */
+spimage_EXPORT int spx_phaser_tiled(SpPhaser *ph);
 
 /* Unnormalised c2c transform of an (nz, ny, nx) x-fastest complex64 array in
  * device memory; direction -1 forward, +1 inverse; d_in == d_out allowed.

@@ -986,15 +986,24 @@ int spx_device_memory(unsigned long long* free_b, unsigned long long* total_b) {
 
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
-int spx_fft_variant(int L, int* row_e, int* col_e) {
+int spx_fft_variant(int L, int tiled, int* row_e, int* col_e) {
   const spx::LengthEntry* e = spx::find_length(L);
   if (!e) {
     spx::set_error("length %d is not supported", L);
     return -11;
   }
-  if (row_e) *row_e = e->var[e->row_choice].E;
-  if (col_e) *col_e = e->var[e->col_choice].E;
+  if (row_e) *row_e = e->var[tiled ? e->row_t4_choice : e->row_choice].E;
+  if (col_e) *col_e = e->var[tiled ? e->col_t4_choice : e->col_choice].E;
   return 0;
+}
+
+int spx_phaser_tiled(SpPhaser* ph) {
+  Engine* e = engine_of(ph);
+  if (!e) {
+    spx::set_error("phaser has no HIP engine");
+    return -1;
+  }
+  return e->tiled() ? 1 : 0;
 }
 
 int spx_fft_device(void* d_in, void* d_out, int nx, int ny, int nz, int direction) {

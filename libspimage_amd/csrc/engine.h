@@ -52,6 +52,8 @@ struct Shape {
   int nx, ny, nz, nrec;
   const LengthEntry *ex, *ey, *ez;
   LineGeom gy, gz;
+  bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
+  LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
 
@@ -75,6 +77,7 @@ class Engine {
   int upload_phased(const float2* phased, const uint8_t* flags);
   int upload_model(const float2* model, int rec = -1);  // into g1
   int download(const float2* dev, float2* host, int rec = -1);
+  bool tiled() const { return sh.t4; }  // Fourier-space arrays in the 4-column tiled layout
   float2* g1() { return g[cur]; }
   float2* g0() { return g[cur ^ 1]; }
   float2* gp_buf() { return gp; }
@@ -99,7 +102,16 @@ class Engine {
   // n iterations (no gp) replayed from HIP graphs of one iteration, one per
   // parity of the ping-pong iterates; recaptured when p or beta change
   int iterate_graph(const IterParams& p, float beta, int n);
-  void invalidate_R() { R_valid = false; Rf_valid = false; }
+  // W (= FFT_x of the current iterate) must be recomputed: for every
+  // reconstruction (rec < 0) or only for `rec`, whose iterate was replaced.
+  // The others keep the W the fused update kernel wrote, so a reconstruction
+  // that shares a batch with newly started ones follows bit for bit the
+  // trajectory it follows alone.
+  void invalidate_R(int rec = -1) {
+    Rf_valid = false;
+    if (rec < 0) R_valid = false;
+    else R_stale.push_back(rec);
+  }
 
   // ---- metrics (reference src/phasing.c:803-893), nrec results each ----
   int efourier(double* out /* 4 per rec */, float2* fmodel_out /* device, may be null */);
@@ -151,6 +163,7 @@ class Engine {
   int* alg_rec = nullptr;     // optional per-rec tables
   float* beta_rec = nullptr;
   bool R_valid = false, Rf_valid = false;
+  std::vector<int> R_stale;  // reconstructions whose W is out of date
   unsigned Rf_version = 0;
   float Rf_gamma1 = 0.f;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

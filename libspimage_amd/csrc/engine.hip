@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -75,47 +76,74 @@ static int variant_index(const LengthEntry* e, int E) {
 
 // Milliseconds per launch of `launch` (median-free: best of a few batches).
 template <class F>
-static float time_launches(hipStream_t s, F&& launch) {
+static float time_launches(hipStream_t s, F&& launch, int warm = 2, int per = 8) {
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return 1e30f;
   if (hipEventCreate(&b) != hipSuccess) {
     (void)hipEventDestroy(a);
     return 1e30f;
   }
-  launch();
-  launch();
+  for (int i = 0; i < warm; ++i) launch();
   float best = 1e30f;
   for (int rep = 0; rep < 3; ++rep) {
     (void)hipEventRecord(a, s);
-    for (int i = 0; i < 8; ++i) launch();
+    for (int i = 0; i < per; ++i) launch();
     (void)hipEventRecord(b, s);
     (void)hipEventSynchronize(b);
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, a, b) == hipSuccess && ms / 8 < best) best = ms / 8;
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess && ms / per < best) best = ms / per;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return (hipGetLastError() == hipSuccess) ? best : 1e30f;
 }
 
-static std::mutex tune_mu;
-static std::map<LengthEntry*, int> tune_done;  // bit 1 row tuned, bit 2 column tuned
+// Tuner scratch: finite non-zero data (a support band, measured pixels) so
+// that the timed passes take the branches and traffic of a real iteration.
+__global__ void k_tune_fill(float2* w, float2* g0, float2* g1, float* amp, uint8_t* fl, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float v = (float)((unsigned)(i * 2654435761u) & 1023u) * (1.f / 1024.f);
+    w[i] = make_float2(v, 0.5f - v);
+    g0[i] = make_float2(0.25f * v, v);
+    g1[i] = make_float2(v, 0.25f * v);
+    amp[i] = 1.f + v;
+    fl[i] = (uint8_t)(((i & 63) < 8 ? 1 : 0) | 2);
+  }
+}
 
-void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
-  std::mutex& mu = tune_mu;
-  std::map<LengthEntry*, int>& done = tune_done;
-  std::lock_guard<std::mutex> lock(mu);
-  const int bit = col_geo ? 2 : 1;
-  if (done[e] & bit) return;
-  done[e] |= bit;
-  int choice = variant_index(e, 8);
-  if (choice < 0) choice = 0;
+static std::mutex tune_mu;
+// per length: bit 1 row tuned, 2 column tuned, 4 tiled row tuned, 8 tiled column tuned
+static std::map<LengthEntry*, int> tune_done;
+
+// variant with E elements per thread (forced by SPX_FFT_E, else 8) among
+// those usable in the layout, else the first usable one
+static int default_choice(const LengthEntry* e, bool need_t4) {
+  const char* force = getenv("SPX_FFT_E");
+  const int pref = force ? atoi(force) : 8;
+  int first = -1;
+  for (int i = 0; i < e->nvar; ++i) {
+    if (need_t4 && !e->var[i].t4) continue;
+    if (e->var[i].E == pref) return i;
+    if (first < 0) first = i;
+  }
+  return first < 0 ? 0 : first;
+}
+
+static bool tuning_off() {
   const char* force = getenv("SPX_FFT_E");
   const char* notune = getenv("SPX_NO_TUNE");
-  if (force && variant_index(e, atoi(force)) >= 0) {
-    choice = variant_index(e, atoi(force));
-  } else if (e->nvar > 1 && !(notune && atoi(notune) == 1)) {
-    // time the hot kernels of the pass on zero-filled scratch of the real geometry
+  return (force && atoi(force) > 0) || (notune && atoi(notune) == 1);
+}
+
+// Single-pass tuning for lengths outside a 2-D pair (3-D axes): times the
+// pass's hot kernel on zero-filled scratch of the real geometry.
+void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
+  std::lock_guard<std::mutex> lock(tune_mu);
+  const int bit = col_geo ? 2 : 1;
+  if (tune_done[e] & bit) return;
+  tune_done[e] |= bit;
+  int choice = default_choice(e, false);
+  if (e->nvar > 1 && !tuning_off()) {
     const long L = e->L;
     long n;
     if (col_geo) n = (long)(col_geo->nouter - 1) * col_geo->outer_stride + (L - 1) * col_geo->axis_stride + col_geo->ncontig;
@@ -154,7 +182,7 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
           u.alg = kER;
           u.scale = 1.f;
           u.nsize = 1.f;
-          ms = time_launches(s, [&] { v.update(u, false, s); });
+          ms = time_launches(s, [&] { v.update(u, false, false, s); });
         }
         if (ms < best) {
           best = ms;
@@ -172,86 +200,124 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
   else e->row_choice = choice;
 }
 
-static void tune_pair(LengthEntry* ex, LengthEntry* ey, int lines, const LineGeom& gy) {
-  {
-    std::lock_guard<std::mutex> lock(tune_mu);
-    const bool need_row = !(tune_done[ex] & 1), need_col = !(tune_done[ey] & 2);
-    const char* force = getenv("SPX_FFT_E");
-    const char* notune = getenv("SPX_NO_TUNE");
-    const bool skip = (force && atoi(force) > 0) || (notune && atoi(notune) == 1) || (ex->nvar < 2 && ey->nvar < 2);
-    if (!need_row && !need_col) return;
-    if (!skip) {
-      tune_done[ex] |= 1;
-      tune_done[ey] |= 2;
-      // both passes in sequence, as one iteration runs them: a variant's
-      // cost depends on where the previous pass left the data
-      const long L = ex->L;
-      const int tl = (long)lines * L > (1L << 24) ? (int)((1L << 24) / L) : lines;  // cap the scratch
-      LineGeom g = gy;
-      if ((long)g.nouter * g.outer_stride > (long)tl * L) g.nouter = (int)(((long)tl * L) / g.outer_stride);
-      if (g.nouter < 1) g.nouter = 1;
-      const int nl = (int)(((long)g.nouter * g.outer_stride) / L);
-      const long n = (long)nl * L;
-      hipStream_t s = nullptr;
-      float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
-      float* amp = nullptr;
-      uint8_t* fl = nullptr;
-      bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
-                hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
-                hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
-                hipMalloc(&fl, (size_t)n) == hipSuccess;
-      ok = ok && hipMemsetAsync(w, 0, sizeof(float2) * n, s) == hipSuccess &&
-           hipMemsetAsync(g0, 0, sizeof(float2) * n, s) == hipSuccess &&
-           hipMemsetAsync(amp, 0, sizeof(float) * n, s) == hipSuccess &&
-           hipMemsetAsync(fl, 0, (size_t)n, s) == hipSuccess;
-      if (ok) {
-        float best = 1e30f;
-        int br = ex->row_choice, bc = ey->col_choice;
-        for (int r = 0; r < ex->nvar; ++r) {
-          if (!need_row && r != ex->row_choice) continue;
-          for (int c = 0; c < ey->nvar; ++c) {
-            if (!need_col && c != ey->col_choice) continue;
-            ColArgs ca{};
-            ca.in = w;
-            ca.W = w;
-            ca.geo = g;
-            ca.ampx = amp;
-            UpdateArgs u{};
-            u.W = w;
-            u.gcur = g0;
-            u.gnew = g1;
-            u.flags = fl;
-            u.nlines = nl;
-            u.lines_per_rec = nl;
-            u.alg = kER;
-            u.scale = 1.f;
-            u.nsize = 1.f;
-            const Variant &vr = ex->var[r], &vc = ey->var[c];
-            const float ms = time_launches(s, [&] {
-              vc.col(kColProject, ca, s);
-              vr.update(u, false, s);
-            });
-            if (ms < best) {
-              best = ms;
-              br = r;
-              bc = c;
-            }
+// Joint tuning of a 2-D shape's row (x) and column (y) passes, timed as one
+// iteration runs them: column pass then row pass, in sequence, because a
+// variant's cost depends on where (which XCD's cache) the previous pass left
+// the data.  Both layouts of the Fourier-space arrays are timed (row-major,
+// and 4-column tiled when nx % 4 == 0 and a tiled row kernel exists); the
+// choice of variants is kept per length and layout, the choice of layout
+// per length pair.  Returns true for the tiled layout.
+static std::map<std::pair<LengthEntry*, LengthEntry*>, bool> tune_layout;
+
+static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_possible) {
+  std::lock_guard<std::mutex> lock(tune_mu);
+  // SPX_NO_T4=1 / SPX_T4=1 force the layout (the tuned choice is kept)
+  const char* not4 = getenv("SPX_NO_T4");
+  const char* yes4 = getenv("SPX_T4");
+  auto forced = [&](bool t4) {
+    if (not4 && atoi(not4) == 1) return false;
+    if (yes4 && atoi(yes4) == 1) return t4_possible;
+    return t4 && t4_possible;
+  };
+  const auto key = std::make_pair(ex, ey);
+  auto it = tune_layout.find(key);
+  if (it != tune_layout.end()) return forced(it->second);
+  float best_t[2] = {1e30f, 1e30f};
+  for (int lay = 0; lay < 2; ++lay) {
+    const bool t4 = lay == 1;
+    if (t4 && !t4_possible) continue;
+    const int rbit = t4 ? 4 : 1, cbit = t4 ? 8 : 2;
+    const bool need_row = !(tune_done[ex] & rbit), need_col = !(tune_done[ey] & cbit);
+    tune_done[ex] |= rbit;
+    tune_done[ey] |= cbit;
+    int& rch = t4 ? ex->row_t4_choice : ex->row_choice;
+    int& cch = t4 ? ey->col_t4_choice : ey->col_choice;
+    if (need_row) rch = default_choice(ex, t4);
+    if (need_col) cch = default_choice(ey, false);
+    if (tuning_off()) continue;
+    const int nx = ex->L;
+    const long N = (long)nx * ny;
+    // cap the scratch at 16 Mi elements (whole reconstructions)
+    int nr = nrec;
+    if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
+    if (nr < 1) nr = 1;
+    LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
+    if (t4) {
+      g.tshift = 2;
+      g.tstride = 4L * ny;
+    }
+    const long n = (long)nr * N;
+    hipStream_t s = nullptr;
+    float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+    float* amp = nullptr;
+    uint8_t* fl = nullptr;
+    bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
+              hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+              hipMalloc(&fl, (size_t)n) == hipSuccess;
+    if (ok) {
+      hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
+      ok = hipGetLastError() == hipSuccess;
+    }
+    if (ok) {
+      int br = rch, bc = cch;
+      for (int r = 0; r < ex->nvar; ++r) {
+        if ((!need_row && r != rch) || (t4 && !ex->var[r].t4)) continue;
+        for (int c = 0; c < ey->nvar; ++c) {
+          if (!need_col && c != cch) continue;
+          ColArgs ca{};
+          ca.in = w;
+          ca.W = w;
+          ca.geo = g;
+          ca.ampx = amp;
+          ca.nt = 1;
+          UpdateArgs u{};
+          u.W = w;
+          u.gcur = g0;
+          u.gnew = g1;
+          u.flags = fl;
+          u.nlines = nr * ny;
+          u.lines_per_rec = ny;
+          u.alg = kHIO;
+          u.beta = 0.9f;
+          u.nt = 1;
+          u.scale = 1.f;
+          u.nsize = 1.f;
+          const Variant &vr = ex->var[r], &vc = ey->var[c];
+          // steady state, iterates ping-ponging as in a run
+          float2 *ga = g0, *gb = g1;
+          const float ms = time_launches(
+              s,
+              [&] {
+                u.gcur = ga;
+                u.gnew = gb;
+                vc.col(kColProject, ca, s);
+                vr.update(u, false, t4, s);
+                std::swap(ga, gb);
+              },
+              20, 30);
+          if (getenv("SPX_TUNE_LOG"))
+            fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr, t4 ? "tiled" : "row-major",
+                    vr.E, vc.E, ms * 1000.f);
+          if (ms < best_t[lay]) {
+            best_t[lay] = ms;
+            br = r;
+            bc = c;
           }
         }
-        ex->row_choice = br;
-        ey->col_choice = bc;
       }
-      (void)hipStreamSynchronize(s);
-      void* bufs[] = {w, g0, g1, amp, fl};
-      for (void* p : bufs)
-        if (p) (void)hipFree(p);
-      if (s) (void)hipStreamDestroy(s);
-      return;
+      rch = br;
+      cch = bc;
     }
+    (void)hipStreamSynchronize(s);
+    void* bufs[] = {w, g0, g1, amp, fl};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
   }
-  // forced or disabled tuning: the per-length rule
-  tune_length(ex, lines, nullptr);
-  tune_length(ey, 0, &gy);
+  const bool t4 = t4_possible && best_t[1] < best_t[0];
+  tune_layout[key] = t4;
+  return forced(t4);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,13 +339,41 @@ __global__ void k_fill_u32(unsigned* p, long n, unsigned v) {
 
 // amplitudes as the projection reads them: unmeasured pixels get the NaN
 // pattern kUnmeasuredBits, measured NaNs the canonical quiet NaN
-__global__ void k_encode_amp(const float* amp, const uint8_t* flags, float* ampx, long n) {
+// row-major index i of reconstruction-stacked (ny, nx) arrays -> index in
+// the 4-column tiled layout
+__device__ __forceinline__ long t4_index(long i, int nx, int ny) {
+  const long N = (long)nx * ny;
+  const long r = i / N, j = i - r * N;
+  const int y = (int)(j / nx), x = (int)(j - (long)y * nx);
+  return r * N + t4_offset(x, ny) + 4L * y;
+}
+
+// encoded amplitudes (unmeasured: kUnmeasuredBits), written row-major
+// (t4 == 0) or tiled
+__global__ void k_encode_amp(const float* amp, const uint8_t* flags, float* ampx, long n, int nx, int ny, int t4) {
   GRID_STRIDE(i, n) {
     float a = amp[i];
     if (!(flags[i] & kMeasured)) a = __uint_as_float(kUnmeasuredBits);
     else if (isnan(a)) a = __uint_as_float(0x7FC00000u);
-    ampx[i] = a;
+    ampx[t4 ? t4_index(i, nx, ny) : i] = a;
   }
+}
+
+// phased amplitudes of the SpRecoverAmplitudes objective; unmeasured pixels
+// carry kUnmeasuredBits in .x
+__global__ void k_encode_phased(const float2* ph, const uint8_t* flags, float2* out, long n, int nx, int ny, int t4) {
+  GRID_STRIDE(i, n) {
+    const float2 v = (flags[i] & kMeasured) ? ph[i] : make_float2(__uint_as_float(kUnmeasuredBits), 0.f);
+    out[t4 ? t4_index(i, nx, ny) : i] = v;
+  }
+}
+
+__global__ void k_layout_f32(const float* in, float* out, long n, int nx, int ny, int t4) {
+  GRID_STRIDE(i, n) out[t4 ? t4_index(i, nx, ny) : i] = in[i];
+}
+
+__global__ void k_untile_c64(const float2* in, float2* out, long n, int nx, int ny) {
+  GRID_STRIDE(i, n) out[i] = in[t4_index(i, nx, ny)];
 }
 
 __global__ void k_scale_copy(const float2* in, float2* out, long n, float s) {
@@ -528,7 +622,7 @@ inline int rec_blocks(long n) {
 // shapes and pass sequences shared by the engine and the one-shot transforms
 // ---------------------------------------------------------------------------
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
-  memset(sh, 0, sizeof(*sh));
+  *sh = Shape{};
   sh->nx = nx;
   sh->ny = ny;
   sh->nz = nz;
@@ -557,8 +651,14 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   // blocks of nx*ny adjacent lines
   sh->gy = LineGeom{nx, nz * nrec, (long)nx * ny, (long)nx};
   sh->gz = LineGeom{nx * ny, nrec, (long)nx * ny * nz, (long)nx * ny};
+  // 2-D shapes may keep their Fourier-space arrays in the 4-column tiled
+  // layout (nx % 4 == 0 and a tiled row kernel for nx); the tuner decides
+  sh->gyt = LineGeom{nx, nrec, (long)nx * ny, 4L};
+  sh->gyt.tshift = 2;
+  sh->gyt.tstride = 4L * ny;
   if (found[0] && found[1] && !found[2]) {
-    tune_pair(found[0], found[1], ny * nrec, sh->gy);
+    const bool t4_possible = nx % 4 == 0 && found[0]->has_t4();
+    sh->t4 = tune_pair(found[0], found[1], ny, nrec, t4_possible);
   } else {
     if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
     if (found[1]) tune_length(found[1], 0, &sh->gy);
@@ -606,8 +706,11 @@ static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir,
 
 // Column work after the x pass: the last non-trivial axis gets `mode`; the
 // middle axis (y of a 3-D array) is transformed forward before and inverse
-// after it.  For kColEFourier nothing is written back.
-static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2* out, ColArgs base, hipStream_t s) {
+// after it.  For kColEFourier nothing is written back.  `tiled`: the arrays
+// are in the shape's 4-column tiled layout (the iteration's W, Wf and
+// Fourier-space inputs when sh.t4); scratch transforms pass false.
+static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2* out, ColArgs base, hipStream_t s,
+                            bool tiled) {
   const bool three = sh.ez && sh.ey;
   ColArgs a = base;
   if (three) {
@@ -627,20 +730,29 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
         a.geo = sh.gy;
       sh.ey->col(kColInv, a, s);
     }
+  } else if (sh.ez) {
+    a.in = in;
+    a.W = out;
+    a.geo = sh.gz;
+    sh.ez->col(mode, a, s);
   } else {
     a.in = in;
     a.W = out;
-    a.geo = sh.ez ? sh.gz : sh.gy;
-    (sh.ez ? sh.ez : sh.ey)->col(mode, a, s);
+    a.geo = tiled ? sh.gyt : sh.gy;
+    sh.ey->col(mode, a, s, tiled);
   }
 }
 
 // column tiles per reconstruction of the last axis (the efourier partials)
 static int last_axis_tiles_per_rec(const Shape& sh) {
-  const LengthEntry* last = sh.ez ? sh.ez : sh.ey;
-  LineGeom g = sh.ez ? sh.gz : sh.gy;
-  g.nouter = sh.ez ? 1 : sh.nz;  // one reconstruction
-  return last->col_blocks(g);
+  if (sh.ez) {
+    LineGeom g = sh.gz;
+    g.nouter = 1;  // one reconstruction
+    return sh.ez->col_blocks(g);
+  }
+  LineGeom g = sh.t4 ? sh.gyt : sh.gy;
+  g.nouter = 1;
+  return sh.ey->col_blocks(g, sh.t4);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,7 +810,7 @@ int Engine::alloc_all() {
   SPX_HIP(hipMemsetAsync(flags, 0, (size_t)NT, s));
   SPX_HIP(hipMemsetAsync(absb, 0, f, s));
   SPX_HIP(hipMemsetAsync(ctl, 0, sizeof(unsigned) * (size_t)kCtlStride * nrec, s));
-  hipLaunchKernelGGL(k_encode_amp, ew_grid(NT), dim3(256), 0, s, absb, flags, ampx, NT);
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(NT), dim3(256), 0, s, absb, flags, ampx, NT, nx, ny, sh.t4 ? 1 : 0);
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -721,7 +833,8 @@ int Engine::upload_amplitudes(const float* amp, const uint8_t* fl, int rec) {
   const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
   SPX_HIP(hipMemcpyAsync(absb + off, amp, sizeof(float) * n, hipMemcpyHostToDevice, s));
   SPX_HIP(hipMemcpyAsync(flags + off, fl, (size_t)n, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_encode_amp, ew_grid(n), dim3(256), 0, s, absb + off, flags + off, ampx + off, n);
+  hipLaunchKernelGGL(k_encode_amp, ew_grid(n), dim3(256), 0, s, absb + off, flags + off, ampx + off, n, nx, ny,
+                     sh.t4 ? 1 : 0);
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
@@ -749,8 +862,12 @@ int Engine::upload_margins(const float* lo, const float* hi) {
   drop_graphs();
   if (!amp_min) SPX_HIP(hipMalloc(&amp_min, sizeof(float) * NT));
   if (!amp_max) SPX_HIP(hipMalloc(&amp_max, sizeof(float) * NT));
-  SPX_HIP(hipMemcpyAsync(amp_min, lo, sizeof(float) * NT, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipMemcpyAsync(amp_max, hi, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+  // staged through the |blur| scratch into the Fourier-space layout
+  SPX_HIP(hipMemcpyAsync(absb, lo, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_layout_f32, ew_grid(NT), dim3(256), 0, s, absb, amp_min, NT, nx, ny, sh.t4 ? 1 : 0);
+  SPX_HIP(hipMemcpyAsync(absb, hi, sizeof(float) * NT, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_layout_f32, ew_grid(NT), dim3(256), 0, s, absb, amp_max, NT, nx, ny, sh.t4 ? 1 : 0);
+  SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -758,8 +875,10 @@ int Engine::upload_margins(const float* lo, const float* hi) {
 int Engine::upload_phased(const float2* ph, const uint8_t* fl) {
   drop_graphs();
   if (!phased) SPX_HIP(hipMalloc(&phased, sizeof(float2) * NT));
-  SPX_HIP(hipMemcpyAsync(phased, ph, sizeof(float2) * NT, hipMemcpyHostToDevice, s));
+  SPX_HIP(hipMemcpyAsync(S1, ph, sizeof(float2) * NT, hipMemcpyHostToDevice, s));
   SPX_HIP(hipMemcpyAsync(flags, fl, (size_t)NT, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_encode_phased, ew_grid(NT), dim3(256), 0, s, S1, flags, phased, NT, nx, ny, sh.t4 ? 1 : 0);
+  SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
   return 0;
@@ -769,7 +888,7 @@ int Engine::upload_model(const float2* model, int rec) {
   const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
   SPX_HIP(hipMemcpyAsync(g[cur] + off, model, sizeof(float2) * n, hipMemcpyHostToDevice, s));
   SPX_HIP(hipStreamSynchronize(s));
-  invalidate_R();
+  invalidate_R(rec);
   return 0;
 }
 
@@ -863,23 +982,42 @@ int Engine::finish_model_init(int fl, int rec) {
   hipLaunchKernelGGL(k_fill_c64, ew_grid(n), dim3(256), 0, s, gp + off, n, make_float2(0.f, 0.f));
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
-  invalidate_R();
+  invalidate_R(rec);
   return 0;
 }
 
 int Engine::ensure_R() {
-  if (R_valid) return 0;
-  sh.ex->row(kRowFwd, row_args(sh, g[cur], W), s);
+  if (!R_valid) {
+    if (sh.t4) sh.ex->row_t4(row_args(sh, g[cur], W), s);
+    else sh.ex->row(kRowFwd, row_args(sh, g[cur], W), s);
+    R_valid = true;
+    R_stale.clear();
+  } else if (!R_stale.empty()) {
+    // one reconstruction's lines at a time (its own tiled block when t4)
+    std::sort(R_stale.begin(), R_stale.end());
+    R_stale.erase(std::unique(R_stale.begin(), R_stale.end()), R_stale.end());
+    for (int r : R_stale) {
+      const long off = (long)r * N;
+      if (sh.t4) one.ex->row_t4(row_args(one, g[cur] + off, W + off), s);
+      else one.ex->row(kRowFwd, row_args(one, g[cur] + off, W + off), s);
+    }
+    R_stale.clear();
+  }
   SPX_HIP(hipGetLastError());
-  R_valid = true;
   return 0;
 }
 
 int Engine::ensure_Rf(float gamma1) {
   if (!Wf) SPX_HIP(hipMalloc(&Wf, sizeof(float2) * NT));
   if (Rf_valid && Rf_version == flags_version && Rf_gamma1 == gamma1) return 0;
-  hipLaunchKernelGGL(k_dm_f1, ew_grid(NT), dim3(256), 0, s, g[cur], flags, gamma1, Wf, NT);
-  sh.ex->row(kRowFwd, row_args(sh, Wf, Wf), s);
+  if (sh.t4) {
+    // the tiled transform cannot run in place: f1 goes through the scratch
+    hipLaunchKernelGGL(k_dm_f1, ew_grid(NT), dim3(256), 0, s, g[cur], flags, gamma1, S1, NT);
+    sh.ex->row_t4(row_args(sh, S1, Wf), s);
+  } else {
+    hipLaunchKernelGGL(k_dm_f1, ew_grid(NT), dim3(256), 0, s, g[cur], flags, gamma1, Wf, NT);
+    sh.ex->row(kRowFwd, row_args(sh, Wf, Wf), s);
+  }
   SPX_HIP(hipGetLastError());
   Rf_valid = true;
   Rf_version = flags_version;
@@ -905,10 +1043,10 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   ca.amp_min = amp_min;
   ca.amp_max = amp_max;
   ca.phased_amp = phased;
-  ca.flags = flags;
   ca.centro = (p.constraints & kCentrosymmetricObject) ? 1 : 0;
   ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
   ca.objective = dm ? 0 : p.objective;
+  ca.nt = (nz == 1) ? 1 : 0;
   ColArgs caf = ca;
   caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
   UpdateArgs u{};
@@ -925,16 +1063,17 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.gamma2 = p.gamma2;
   u.scale = (float)(1.0 / (double)N);
   u.nsize = (float)N;
+  u.nt = (nz == 1) ? 1 : 0;
   for (int i = 0; i < n; ++i) {
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
-    enqueue_fourier(sh, kColProject, W, W, ca, s);
-    if (dm) enqueue_fourier(sh, kColProject, Wf, Wf, caf, s);
+    enqueue_fourier(sh, kColProject, W, W, ca, s, sh.t4);
+    if (dm) enqueue_fourier(sh, kColProject, Wf, Wf, caf, s, sh.t4);
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i + 1], s));
     u.gcur = g[cur];
     u.gnew = g[cur ^ 1];
     u.gp = (want_gp && i == n - 1) ? gp : nullptr;
     u.beta = betas ? betas[i] : 0.f;
-    sh.ex->update(u, dm, s);
+    sh.ex->update(u, dm, sh.t4, s);
     cur ^= 1;
   }
   if (marks) SPX_HIP(hipEventRecord(marks[2 * n], s));
@@ -1051,14 +1190,19 @@ int Engine::efourier(double* out, float2* fout) {
   ca.partials = partials;
   ca.fout = fout;
   // the 3-D path needs somewhere to put the y-transformed lines: S1
-  enqueue_fourier(sh, kColEFourier, W, S1, ca, s);
+  enqueue_fourier(sh, kColEFourier, W, S1, ca, s, sh.t4);
   SPX_HIP(hipGetLastError());
   return reduce_per_rec(last_axis_tiles_per_rec(sh), 4, out);
 }
 
 int Engine::fmodel(float2* fout) {
   std::vector<double> dummy((size_t)4 * nrec);
-  return efourier(dummy.data(), fout);
+  if (!sh.t4) return efourier(dummy.data(), fout);
+  // the transform comes out in the tiled layout: untile it into `fout`
+  if (int r = efourier(dummy.data(), S1)) return r;
+  hipLaunchKernelGGL(k_untile_c64, ew_grid(NT), dim3(256), 0, s, S1, fout, NT, nx, ny);
+  SPX_HIP(hipGetLastError());
+  return 0;
 }
 
 int Engine::ereal(double* out) {
@@ -1090,7 +1234,7 @@ int Engine::blur_to_abs(float sigma, bool with_hist) {
   cb.ny = ny;
   cb.nz = nz;
   cb.sigma = sigma;
-  enqueue_fourier(sh, kColBlur, S1, S1, cb, s);
+  enqueue_fourier(sh, kColBlur, S1, S1, cb, s, false);
   RowArgs r = row_args(sh, S1, nullptr);
   r.absb = absb;
   r.hist = with_hist ? hist : nullptr;
@@ -1202,7 +1346,7 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
       cb.ny = ny;
       cb.nz = nz;
       cb.sigma = sigma;
-      enqueue_fourier(sh, kColBlur, d, d, cb, s);
+      enqueue_fourier(sh, kColBlur, d, d, cb, s, false);
       sh.ex->row(kRowInv, row_args(sh, d, d), s);
     }
   }

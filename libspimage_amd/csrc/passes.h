@@ -41,6 +41,28 @@ enum : int {
   kPositivityFlipping = 8, kCentrosymmetricObject = 16, kAmplitudeErrorMargin = 32
 };
 
+// In 2-D iterations the row-transformed iterate W crosses XCDs between the
+// two passes (rows are spread over XCDs in one pass, columns in the other),
+// so those passes load and store it with the non-temporal hint (nt = 1): it
+// streams through L2 instead of displacing the per-XCD data that is reused
+// (iterates, flags, amplitudes) and leaves fewer dirty lines for the
+// end-of-kernel write-back.  3-D passes keep ordinary accesses (measured
+// faster there).
+typedef float spx_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  const spx_v2f x = {v.x, v.y};
+  __builtin_nontemporal_store(x, reinterpret_cast<spx_v2f*>(p));
+}
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+  const spx_v2f x = __builtin_nontemporal_load(reinterpret_cast<const spx_v2f*>(p));
+  return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ float2 ld_line(const float2* p, int nt) { return nt ? ld_nt(p) : *p; }
+__device__ __forceinline__ void st_line(float2* p, float2 v, int nt) {
+  if (nt) st_nt(p, v);
+  else *p = v;
+}
+
 // Map hardware work-group order (round-robin over the 8 XCDs) onto tiles so
 // that each XCD receives a contiguous range: neighbouring column groups share
 // 128-byte lines and then share one L2.
@@ -54,12 +76,25 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // Argument blocks (passed by value).
 // ---------------------------------------------------------------------------
 struct LineGeom {
-  // line element i of line (outer, c) lives at outer*outer_stride + c + i*axis_stride
-  int ncontig;       // lines adjacent in memory (x extent for column passes)
+  // line element i of line (outer, c) lives at
+  //   outer*outer_stride + (c >> tshift)*tstride + (c & (2^tshift - 1)) + i*axis_stride
+  // natural layout: tshift large (one tile); 4-column tiled layout: tshift 2
+  int ncontig;       // lines of one outer index (x extent for column passes)
   int nouter;        // number of outer indices
   long outer_stride;
   long axis_stride;
+  int tshift = 30;
+  long tstride = 0;
 };
+
+// 4-column tiled layout of a 2-D array with `ny` rows of L elements (one
+// reconstruction after another): element (y, x) sits at
+// (x / 4) * 4 ny + 4 y + x % 4, so a column pass reads each group of 4
+// columns as one contiguous block and a row pass that holds 4 consecutive
+// rows writes whole 128-byte lines.  The engine keeps its Fourier-space
+// arrays (the row-transformed iterate, amplitudes, margins, phased
+// amplitudes) in this layout for 2-D shapes with nx % 4 == 0.
+__device__ __forceinline__ long t4_offset(int x, int ny) { return (long)(x >> 2) * (4L * ny) + (x & 3); }
 
 enum RowMode : int {
   kRowFwd = 0,        // out = FFT_x(in)
@@ -104,6 +139,7 @@ struct UpdateArgs {
   float gamma1, gamma2;
   float scale;             // 1/N as float
   float nsize;             // N as float (diff map divides by it)
+  int nt;                  // W / Wf with the non-temporal hint
 };
 
 enum ColMode : int {
@@ -123,8 +159,7 @@ struct ColArgs {
   const float* ampx;         // encoded amplitudes (kUnmeasuredBits for unmeasured)
   const float* amp_min;
   const float* amp_max;
-  const float2* phased_amp;  // SpRecoverAmplitudes objective
-  const uint8_t* flags;      // read only by the phased-amplitude objective
+  const float2* phased_amp;  // SpRecoverAmplitudes objective (unmeasured: kUnmeasuredBits in .x)
   int centro;                // apply the centrosymmetric Fourier constraint
   int margin;                // SpAmplitudeErrorMargin
   int objective;             // 0 phases, 1 amplitudes
@@ -134,6 +169,7 @@ struct ColArgs {
   // blur
   int nx, ny, nz;
   float sigma;
+  int nt;                    // line data with the non-temporal hint
 };
 
 // ---------------------------------------------------------------------------
@@ -300,30 +336,65 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
 // the difference map, of its f1) for the next iteration's column pass.  The
 // update runs on registers at the inverse transform's output positions; its
 // other operands (g_k, flags) are loaded before the inverse transform.
-template <int L, int E, bool DM>
-__global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs a) {
+// Thread layout of row work-groups.  Natural layout: M lines of T threads
+// each (thread = line * T + t).  4-column tiled layout of W: 4 consecutive
+// lines whose lanes interleave as 16 positions x 4 lines, so that one
+// wavefront load covers 16 consecutive x of 4 rows: four whole 128-byte
+// lines of a row-major array and four whole 4 x 4 tiles of the tiled one.
+template <int L, int E, bool T4>
+struct RowGeo {
+  static constexpr int T = PlanOf<L, E>::T;
+  static constexpr int M = T4 ? 4 : RowShape<L, E>::M;
+  static constexpr int NT = T * M;
+  __device__ static __forceinline__ int line_of(int tid) { return T4 ? ((tid >> 4) & 3) : tid / T; }
+  __device__ static __forceinline__ int pos_of(int tid) {
+    return T4 ? (((tid >> 6) << 4) | (tid & 15)) : tid - (tid / T) * T;
+  }
+};
+// the tiled row kernels need T % 16 == 0, 4 T <= 1024 threads and two LDS
+// buffers of 4 lines within 96 KiB
+template <int L, int E>
+struct T4Ok {
+  static constexpr int T = PlanOf<L, E>::T;
+  static constexpr bool value = (T % 16 == 0) && (4 * T <= 1024) && (2 * LdsView<L, 4, false>::SIZE * 8 <= 98304);
+};
+
+template <int L, int E, bool DM, bool T4>
+__global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArgs a) {
   using PL = PlanOf<L, E>;
-  constexpr int T = RowShape<L, E>::T, M = RowShape<L, E>::M, NT = RowShape<L, E>::NT;
+  using RG = RowGeo<L, E, T4>;
+  constexpr int M = RG::M;
   constexpr int NR = PL::NREG;
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[View::SIZE];
   const int tid = threadIdx.x;
-  const int g = tid / T, t = tid - g * T;
+  const int g = RG::line_of(tid), t = RG::pos_of(tid);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
   const bool act = line < a.nlines;
   const View va{shA, g}, vb{shB, g};
   // idle lanes of a partial last work-group address the last line (loads
   // stay in bounds and need no predicate); only their stores are skipped
-  const long base = (long)(act ? line : a.nlines - 1) * L;
+  const int lc = act ? line : a.nlines - 1;
+  const long base = (long)lc * L;
+  // W (and Wf): natural rows, or the 4-column tiled layout
+  long wbase = base;
+  if constexpr (T4) {
+    const int rec = lc / a.lines_per_rec, y = lc - rec * a.lines_per_rec;
+    wbase = (long)rec * L * a.lines_per_rec + 4L * y;
+  }
+  auto wpos = [&](int x) -> long {
+    if constexpr (T4) return t4_offset(x, a.lines_per_rec);
+    else return x;
+  };
 
   float2 v[NR], vf[DM ? NR : 1], g0v[NR];
   uint8_t fl[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    v[q] = a.W[base + in_pos<L, E>(t, q)];
-    if constexpr (DM) vf[q] = a.Wf[base + in_pos<L, E>(t, q)];
+    v[q] = ld_line(&a.W[wbase + wpos(in_pos<L, E>(t, q))], a.nt);
+    if constexpr (DM) vf[q] = ld_line(&a.Wf[wbase + wpos(in_pos<L, E>(t, q))], a.nt);
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
@@ -350,7 +421,6 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
   const float cg = alg == kHIO ? 1.f : beta;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    if (!act) continue;
     const long idx = base + out_pos<L, E>(t, q);
     const float2 P = v[q];
     const float2 g0 = g0v[q];
@@ -358,14 +428,14 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
     float2 g1;
     if constexpr (!DM) {
       const float2 gp = make_float2(P.x * a.scale, P.y * a.scale);
-      if (a.gp) a.gp[idx] = gp;
+      if (a.gp && act) a.gp[idx] = gp;
       float2 o = make_float2(fmaf(cp, gp.x, cg * g0.x), fmaf(cp, gp.y, cg * g0.y));
       if (er) o = make_float2(0.f, 0.f);
       const float2 gi = apply_constraints(gp, a.constraints);
       g1 = make_float2(in ? gi.x : o.x, in ? gi.y : o.y);
     } else {
       // src/phasing.c:1243-1252; gp stays unscaled like the CPU engine
-      if (a.gp) a.gp[idx] = P;
+      if (a.gp && act) a.gp[idx] = P;
       const float2 Pf = vf[q];
       if (in) {
         g1.x = g0.x + beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x);
@@ -375,11 +445,9 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
       }
       g1.x -= beta * Pf.x / a.nsize;
       g1.y -= beta * Pf.y / a.nsize;
-    }
-    if constexpr (DM) {
       if (in) g1 = apply_constraints(g1, a.constraints);
     }
-    a.gnew[idx] = g1;
+    if (act) a.gnew[idx] = g1;
     v[q] = g1;
     // f1 of the next iteration (src/phasing.c:1260-1274)
     if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
@@ -392,11 +460,43 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row_update(UpdateArgs 
     __syncthreads();
     fft_line<L, E, -1>(vf, va, vb, t, twr);
   }
+  if (act) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    if (!act) continue;
-    a.W[base + out_pos<L, E>(t, q)] = v[q];
-    if constexpr (DM) a.Wf[base + out_pos<L, E>(t, q)] = vf[q];
+    for (int q = 0; q < NR; ++q) {
+      st_line(&a.W[wbase + wpos(out_pos<L, E>(t, q))], v[q], a.nt);
+      if constexpr (DM) st_line(&a.Wf[wbase + wpos(out_pos<L, E>(t, q))], vf[q], a.nt);
+    }
+  }
+}
+
+// Forward x-transform of row-major `in` into the 4-column tiled `out`
+// (in != out): the row-transformed iterate W of a 2-D shape.
+template <int L, int E>
+__global__ __launch_bounds__((RowGeo<L, E, true>::NT)) void k_row_fwd_t4(RowArgs a) {
+  using RG = RowGeo<L, E, true>;
+  constexpr int NR = PlanOf<L, E>::NREG;
+  using View = LdsView<L, 4, false>;
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[View::SIZE];
+  const int tid = threadIdx.x;
+  const int g = RG::line_of(tid), t = RG::pos_of(tid);
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int line = blk * 4 + g;
+  const bool act = line < a.nlines;
+  const View va{shA, g}, vb{shB, g};
+  const int lc = act ? line : a.nlines - 1;
+  const long base = (long)lc * L;
+  const int rec = lc / a.lines_per_rec, y = lc - rec * a.lines_per_rec;
+  const long wbase = (long)rec * L * a.lines_per_rec + 4L * y;
+  float2 v[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = a.in[base + in_pos<L, E>(t, q)];
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  fft_line<L, E, -1>(v, va, vb, t, twr);
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) st_nt(&a.out[wbase + t4_offset(out_pos<L, E>(t, q), a.lines_per_rec)], v[q]);
   }
 }
 
@@ -435,13 +535,15 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
   const int x = (blk - outer * ngrp) * C + c;
   const bool act = x < a.geo.ncontig;
   // idle lanes address the last line of the group (in bounds, no predicate)
-  const long base = (long)outer * a.geo.outer_stride + (act ? x : a.geo.ncontig - 1);
+  const int xc = act ? x : a.geo.ncontig - 1;
+  const long base = (long)outer * a.geo.outer_stride + (long)(xc >> a.geo.tshift) * a.geo.tstride +
+                    (xc & ((1 << a.geo.tshift) - 1));
   const long st = a.geo.axis_stride;
   const View va{shA, c}, vb{shB, c};
 
   float2 v[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) v[q] = a.in[base + in_pos<L, E>(t, q) * st];
+  for (int q = 0; q < NR; ++q) v[q] = ld_line(&a.in[base + in_pos<L, E>(t, q) * st], a.nt);
   // amplitudes of the projection, fetched with the data
   float ampv[(MODE == kColProject || MODE == kColEFourier) ? NR : 1];
   if constexpr (MODE == kColProject || MODE == kColEFourier) {
@@ -464,8 +566,11 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
       if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
       if (a.objective == 0) {
         G = project_pixel(G, ampv[q], a, idx);
-      } else if (a.flags[idx] & kMeasured) {
-        G = a.phased_amp[idx];  // src/phasing.c:1103-1109
+      } else {
+        // src/phasing.c:1103-1109: measured pixels take the phased
+        // amplitude; unmeasured entries carry kUnmeasuredBits in .x
+        const float2 P = a.phased_amp[idx];
+        if (__float_as_uint(P.x) != kUnmeasuredBits) G = P;
       }
       v[q] = G;
     }
@@ -532,7 +637,7 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
 
 #pragma unroll
   for (int q = 0; q < NR; ++q)
-    if (act) a.W[base + out_pos<L, E>(t, q) * st] = v[q];
+    if (act) st_line(&a.W[base + out_pos<L, E>(t, q) * st], v[q], a.nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,12 +663,28 @@ struct LengthOps {
       case kRowBlurOut: hipLaunchKernelGGL((k_row<L, E, kRowBlurOut>), grid, blk, 0, s, a); break;
     }
   }
-  static void update(const UpdateArgs& a0, bool dm, hipStream_t s) {
+  static constexpr bool t4 = T4Ok<L, E>::value;
+  static void update(const UpdateArgs& a0, bool dm, bool tiled, hipStream_t s) {
     UpdateArgs a = a0;
     a.tw = tw();
+    if (tiled) {
+      if constexpr (t4) {
+        const dim3 grid((a.nlines + 3) / 4), blk(RowGeo<L, E, true>::NT);
+        if (dm) hipLaunchKernelGGL((k_row_update<L, E, true, true>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_row_update<L, E, false, true>), grid, blk, 0, s, a);
+      }
+      return;
+    }
     const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
-    if (dm) hipLaunchKernelGGL((k_row_update<L, E, true>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_row_update<L, E, false>), grid, blk, 0, s, a);
+    if (dm) hipLaunchKernelGGL((k_row_update<L, E, true, false>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_row_update<L, E, false, false>), grid, blk, 0, s, a);
+  }
+  static void row_t4(const RowArgs& a0, hipStream_t s) {
+    if constexpr (t4) {
+      RowArgs a = a0;
+      a.tw = tw();
+      hipLaunchKernelGGL((k_row_fwd_t4<L, E>), dim3((a.nlines + 3) / 4), dim3(RowGeo<L, E, true>::NT), 0, s, a);
+    }
   }
   static int col_blocks(const LineGeom& g) {
     return ((g.ncontig + ColShape<L, E>::C - 1) / ColShape<L, E>::C) * g.nouter;
@@ -585,30 +706,48 @@ struct LengthOps {
 // One compiled variant (plan with E elements per thread at the edges).
 struct Variant {
   int E;
+  bool t4;  // has the tiled-layout row kernels
   void (*row)(int, const RowArgs&, hipStream_t);
-  void (*update)(const UpdateArgs&, bool, hipStream_t);
+  void (*update)(const UpdateArgs&, bool, bool, hipStream_t);
+  void (*row_t4)(const RowArgs&, hipStream_t);
   void (*col)(int, const ColArgs&, hipStream_t);
   int (*col_blocks)(const LineGeom&);
 };
 
 template <int L, int E>
 constexpr Variant variant() {
-  return Variant{E, &LengthOps<L, E>::row, &LengthOps<L, E>::update, &LengthOps<L, E>::col,
-                 &LengthOps<L, E>::col_blocks};
+  return Variant{E, LengthOps<L, E>::t4, &LengthOps<L, E>::row, &LengthOps<L, E>::update,
+                 &LengthOps<L, E>::row_t4, &LengthOps<L, E>::col, &LengthOps<L, E>::col_blocks};
 }
 
 // Dispatch table entry for one length: its compiled variants and the ones in
-// use for row passes and for column passes (chosen by the tuner).
+// use (chosen by the tuner) for row and column passes, separately for the
+// natural layout and for 2-D shapes in the 4-column tiled layout.
 struct LengthEntry {
   int L;
   int nvar;
   Variant var[3];
   int row_choice;
   int col_choice;
+  int row_t4_choice;
+  int col_t4_choice;
+  bool has_t4() const {
+    for (int i = 0; i < nvar; ++i)
+      if (var[i].t4) return true;
+    return false;
+  }
   void row(int mode, const RowArgs& a, hipStream_t s) const { var[row_choice].row(mode, a, s); }
-  void update(const UpdateArgs& a, bool dm, hipStream_t s) const { var[row_choice].update(a, dm, s); }
-  void col(int mode, const ColArgs& a, hipStream_t s) const { var[col_choice].col(mode, a, s); }
-  int col_blocks(const LineGeom& g) const { return var[col_choice].col_blocks(g); }
+  void update(const UpdateArgs& a, bool dm, bool tiled, hipStream_t s) const {
+    if (tiled) var[row_t4_choice].update(a, dm, true, s);
+    else var[row_choice].update(a, dm, false, s);
+  }
+  void row_t4(const RowArgs& a, hipStream_t s) const { var[row_t4_choice].row_t4(a, s); }
+  void col(int mode, const ColArgs& a, hipStream_t s, bool tiled = false) const {
+    var[tiled ? col_t4_choice : col_choice].col(mode, a, s);
+  }
+  int col_blocks(const LineGeom& g, bool tiled = false) const {
+    return var[tiled ? col_t4_choice : col_choice].col_blocks(g);
+  }
 };
 
 // Registry implemented by the instantiation units (mutable: the tuner sets

@@ -237,18 +237,20 @@ _bind("sp_gaussian_blur", PImage, PImage, C.c_float)
 _bind("spx_device_available", C.c_int)
 _bind("spx_last_error", C.c_char_p)
 _bind("spx_supported_length", C.c_int, C.c_int)
-_bind("spx_fft_variant", C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_bind("spx_fft_variant", C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
 
 
-def fft_variant(L: int) -> tuple[int, int]:
-    """(row, column) elements-per-thread of the tuned transform of length L."""
+def fft_variant(L: int, tiled: bool = False) -> tuple[int, int]:
+    """(row, column) elements-per-thread of the tuned transform of length L
+    in the row-major or the tiled layout."""
     r, c = C.c_int(), C.c_int()
-    if spx_fft_variant(L, C.byref(r), C.byref(c)) != 0:
+    if spx_fft_variant(L, 1 if tiled else 0, C.byref(r), C.byref(c)) != 0:
         raise ValueError(last_error())
     return r.value, c.value
 _bind("spx_fft_device", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float))
 _bind("spx_phaser_time_passes", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float))
+_bind("spx_phaser_tiled", C.c_int, PPhaser)
 _bind("spx_phaser_device_arrays", C.c_int, PPhaser, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
       C.POINTER(C.c_void_p))
 _bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)

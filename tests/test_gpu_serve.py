@@ -30,9 +30,11 @@ def _requests(shapes_algs_its, seed0=500):
     return out
 
 
+# 64 x 64 and 48 x 48 run row-major, 128 x 128 in the tiled layout, 16^3 is 3-D
 MIX = [((64, 64), "hio", 100), ((48, 48), "er", 50), ((64, 64), "raar", 150), ((16, 16, 16), "hio", 100),
        ((64, 64), "er", 50), ((48, 48), "hio", 150), ((64, 64), "hio", 100), ((16, 16, 16), "raar", 50),
-       ((64, 64), "raar", 100), ((48, 48), "raar", 100), ((64, 64), "hio", 50), ((16, 16, 16), "er", 150)]
+       ((64, 64), "raar", 100), ((48, 48), "raar", 100), ((64, 64), "hio", 50), ((16, 16, 16), "er", 150),
+       ((128, 128), "hio", 100), ((128, 128), "raar", 50), ((128, 128), "er", 150), ((128, 128), "hio", 200)]
 
 
 @pytest.mark.parametrize("realtime", [False, True])
@@ -41,19 +43,20 @@ def test_served_equals_alone(sp, realtime):
     reqs = _requests(MIX)
     # at most 2 slots per 64x64 batch: later requests reuse freed slots and
     # extra batches are created while others run
-    srv = serve.Server(quantum=50, max_pixels_per_batch=2 * 64 * 64, reserve_bytes=0, keep_models=True)
+    srv = serve.Server(quantum=50, max_pixels_per_batch=2 * 128 * 128, reserve_bytes=0, keep_models=True)
     res = srv.run(reqs, realtime=realtime)
     assert sorted(r.rid for r in res) == list(range(len(reqs)))
     if not realtime:
-        # all requests queued at once: the 64x64 class alone needs 3 batches
-        assert srv.batches_created >= 4
+        # all requests queued at once: 64x64 (6 requests, 8 slots per batch)
+        # and 128x128 (4 requests, 2 slots) need several batches
+        assert srv.batches_created >= 5
     srv.close()
     plain = serve.PlainRunner()
     by_id = {r.rid: r for r in res}
     for req in reqs:
         ef, fc, model = plain.run_one(req, keep_model=True)
         got = by_id[req.rid]
-        assert np.array_equal(got.model, model), req.rid
+        assert np.array_equal(got.model, model), (req.rid, req.amplitudes.shape, float(np.abs(got.model - model).max()))
         assert abs(got.efourier - ef) <= 1e-6 * abs(ef), req.rid
         assert abs(got.fcfo - fc) <= 1e-6 * abs(fc), req.rid
         assert got.finished >= got.admitted >= (got.arrival if realtime else 0.0)
