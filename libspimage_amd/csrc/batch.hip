@@ -202,6 +202,11 @@ int spx_batch_set_rec_algorithms(SpxBatch* b, const int* alg_types, const float*
   return b->e->set_rec_params(alg_types, betas);
 }
 
+int spx_batch_set_active(SpxBatch* b, const int* active) {
+  if (!b) return fail(-1, "bad batch arguments");
+  return b->e->set_rec_active(active);
+}
+
 int spx_batch_set_support_schedule(SpxBatch* b, SpSupportArray* sup) {
   if (!b) return fail(-1, "bad batch arguments");
   b->sup = sup;

@@ -84,6 +84,9 @@ class Engine {
   float2* scratch() { return S1; }
   // per-reconstruction algorithm / beta tables (null: uniform)
   int set_rec_params(const int* alg, const float* beta);
+  // per-reconstruction activity (nrec entries, 0 = idle: iterations leave
+  // that reconstruction unchanged); null or all non-zero: every one active
+  int set_rec_active(const int* active);
 
   // ---- model initialisation (reference src/phasing.c:606-724) ----
   // rec < 0: every reconstruction, amp_raw holds NT values and reconstruction
@@ -162,6 +165,8 @@ class Engine {
   unsigned* ctl = nullptr;    // kCtlStride per rec
   int* alg_rec = nullptr;     // optional per-rec tables
   float* beta_rec = nullptr;
+  uint8_t* rec_active = nullptr;
+  bool some_idle = false;
   bool R_valid = false, Rf_valid = false;
   std::vector<int> R_stale;  // reconstructions whose W is out of date
   unsigned Rf_version = 0;

@@ -713,32 +713,39 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
                             bool tiled) {
   const bool three = sh.ez && sh.ey;
   ColArgs a = base;
+  // y lines of a 3-D shape have outer index (rec, z); every other
+  // geometry has outer index rec
   if (three) {
     a.in = in;
     a.W = out;
     a.geo = sh.gy;
+    a.outer_per_rec = sh.nz;
     sh.ey->col(kColFwd, a, s);
     a = base;
     a.in = out;
     a.W = out;
     a.geo = sh.gz;
+    a.outer_per_rec = 1;
     sh.ez->col(mode, a, s);
     if (mode != kColEFourier) {
       a = base;
       a.in = out;
       a.W = out;
-        a.geo = sh.gy;
+      a.geo = sh.gy;
+      a.outer_per_rec = sh.nz;
       sh.ey->col(kColInv, a, s);
     }
   } else if (sh.ez) {
     a.in = in;
     a.W = out;
     a.geo = sh.gz;
+    a.outer_per_rec = 1;
     sh.ez->col(mode, a, s);
   } else {
     a.in = in;
     a.W = out;
     a.geo = tiled ? sh.gyt : sh.gy;
+    a.outer_per_rec = 1;
     sh.ey->col(mode, a, s, tiled);
   }
 }
@@ -819,8 +826,8 @@ Engine::~Engine() {
   // teardown: errors here have nowhere to go
   if (s) (void)hipStreamSynchronize(s);
   drop_graphs();
-  void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min, amp_max,
-                  absb, phased, flags, partials, red, hist, ctl, alg_rec, beta_rec};
+  void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min,  amp_max,   absb,
+                  phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -912,6 +919,27 @@ int Engine::set_rec_params(const int* alg, const float* beta) {
     SPX_HIP(hipMemcpyAsync(beta_rec, beta, sizeof(float) * nrec, hipMemcpyHostToDevice, s));
   }
   SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::set_rec_active(const int* active) {
+  bool idle = false;
+  std::vector<uint8_t> h((size_t)nrec, 1);
+  if (active)
+    for (int r = 0; r < nrec; ++r) {
+      h[r] = active[r] ? 1 : 0;
+      if (!active[r]) idle = true;
+    }
+  if (idle) {
+    if (!rec_active) {
+      drop_graphs();  // a new kernel argument
+      SPX_HIP(hipMalloc(&rec_active, (size_t)nrec));
+    }
+    SPX_HIP(hipMemcpyAsync(rec_active, h.data(), (size_t)nrec, hipMemcpyHostToDevice, s));
+    SPX_HIP(hipStreamSynchronize(s));
+  }
+  if (idle != some_idle) drop_graphs();
+  some_idle = idle;
   return 0;
 }
 
@@ -1047,6 +1075,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
   ca.objective = dm ? 0 : p.objective;
   ca.nt = (nz == 1) ? 1 : 0;
+  ca.rec_active = some_idle ? rec_active : nullptr;
   ColArgs caf = ca;
   caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
   UpdateArgs u{};
@@ -1064,6 +1093,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.scale = (float)(1.0 / (double)N);
   u.nsize = (float)N;
   u.nt = (nz == 1) ? 1 : 0;
+  u.rec_active = some_idle ? rec_active : nullptr;
   for (int i = 0; i < n; ++i) {
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
     enqueue_fourier(sh, kColProject, W, W, ca, s, sh.t4);

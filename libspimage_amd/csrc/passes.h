@@ -131,6 +131,7 @@ struct UpdateArgs {
   const float2* tw;
   const float* beta_rec;   // per-reconstruction beta, or null for `beta`
   const int* alg_rec;      // per-reconstruction algorithm, or null for `alg`
+  const uint8_t* rec_active;  // per-reconstruction 0 = idle slot (left unchanged), or null
   int nlines;
   int lines_per_rec;
   int alg;
@@ -170,6 +171,10 @@ struct ColArgs {
   int nx, ny, nz;
   float sigma;
   int nt;                    // line data with the non-temporal hint
+  // idle reconstructions (0 = skip the work-group), or null; outer index /
+  // outer_per_rec is the reconstruction
+  const uint8_t* rec_active;
+  int outer_per_rec;
 };
 
 // ---------------------------------------------------------------------------
@@ -372,11 +377,32 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
   const int g = RG::line_of(tid), t = RG::pos_of(tid);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
-  const bool act = line < a.nlines;
+  bool act = line < a.nlines;
+  bool idle_line = false;
+  if (a.rec_active) {
+    // idle reconstructions keep their state: W stays in place and the
+    // iterate is carried over to the other ping-pong buffer; a work-group
+    // whose lines all belong to idle ones does only that copy and leaves
+    // before its first barrier
+    bool any = false;
+    for (int k = 0; k < M; ++k) {
+      const int l = blk * M + k;
+      if (l < a.nlines && a.rec_active[l / a.lines_per_rec]) any = true;
+    }
+    if (!any) {
+      for (int i = tid; i < M * L; i += RG::NT) {
+        const int l = blk * M + i / L;
+        if (l < a.nlines) a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
+      }
+      return;
+    }
+    idle_line = act && !a.rec_active[line / a.lines_per_rec];
+    act = act && !idle_line;
+  }
   const View va{shA, g}, vb{shB, g};
   // idle lanes of a partial last work-group address the last line (loads
   // stay in bounds and need no predicate); only their stores are skipped
-  const int lc = act ? line : a.nlines - 1;
+  const int lc = line < a.nlines ? line : a.nlines - 1;
   const long base = (long)lc * L;
   // W (and Wf): natural rows, or the 4-column tiled layout
   long wbase = base;
@@ -448,6 +474,7 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
       if (in) g1 = apply_constraints(g1, a.constraints);
     }
     if (act) a.gnew[idx] = g1;
+    else if (idle_line) a.gnew[idx] = g0;
     v[q] = g1;
     // f1 of the next iteration (src/phasing.c:1260-1274)
     if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
@@ -532,6 +559,7 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
   const int ngrp = (a.geo.ncontig + C - 1) / C;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int outer = blk / ngrp;
+  if (MODE != kColEFourier && a.rec_active && !a.rec_active[outer / a.outer_per_rec]) return;
   const int x = (blk - outer * ngrp) * C + c;
   const bool act = x < a.geo.ncontig;
   // idle lanes address the last line of the group (in bounds, no predicate)

@@ -21,6 +21,8 @@ Scheduling:
   estimated footprint (spx_batch_estimate_bytes) fits into the free device
   memory minus a reserve; otherwise idle batches of other classes are
   released first and the requests wait;
+* idle slots cost nothing: the batch's activity mask (spx_batch_set_active)
+  makes the pass kernels skip the work-groups of slots without a request;
 * every batch owns a HIP stream, so quanta of different classes overlap on
   the device; the host loop only polls (spx_batch_query) and never waits
   for one class while another could be fed.
@@ -163,7 +165,13 @@ class Server:
             changed = True
         if changed:
             bt.batch.set_rec_algorithms(bt.algs, bt.betas)
+            self._sync_active(bt)
         return changed
+
+    @staticmethod
+    def _sync_active(bt: _Batch):
+        # idle slots are skipped by the pass kernels
+        bt.batch.set_active([0 if s is None else 1 for s in bt.slots])
 
     # -- progress ------------------------------------------------------------
     def _finish_quantum(self, bt: _Batch, now_fn):
@@ -180,6 +188,7 @@ class Server:
             model = bt.batch.model(r) if self.keep_models else None
             self.results.append(Result(s.req.rid, float(ef[r]), float(fc[r]), s.req.arrival, s.admitted, t, model))
             bt.slots[r] = None
+        self._sync_active(bt)
         if all(s is None for s in bt.slots):
             bt.idle_since = time.perf_counter()
 

@@ -282,6 +282,7 @@ _bind("spx_batch_get_support", C.c_int, PBatch, C.c_int, PI)
 _bind("spx_batch_time_iterations", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_synchronize", C.c_int, PBatch)
 _bind("spx_batch_init_model_rec", C.c_int, PBatch, C.c_int, C.c_int, C.c_ulonglong)
+_bind("spx_batch_set_active", C.c_int, PBatch, C.POINTER(C.c_int))
 _bind("spx_batch_query", C.c_int, PBatch)
 _bind("spx_batch_estimate_bytes", C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_device_memory", C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
@@ -404,6 +405,12 @@ class Batch:
 
     def synchronize(self):
         self._check(spx_batch_synchronize(self.h), "synchronize")
+
+    def set_active(self, active=None):
+        """nrec flags (0 = idle slot, left unchanged by iterations), or None for all."""
+        a = None if active is None else np.ascontiguousarray(active, np.int32)
+        self._keep.append(a)
+        self._check(spx_batch_set_active(self.h, None if a is None else _iptr(a)), "set_active")
 
     def init_model_rec(self, rec, flags, seed):
         self._check(spx_batch_init_model_rec(self.h, rec, flags, seed), "init_model_rec")

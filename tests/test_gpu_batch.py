@@ -124,3 +124,26 @@ def test_batch_random_start_streams_differ(sp):
     m = b.model()
     assert not np.allclose(m[0], m[1])
     b.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (16, 16, 16)])
+def test_idle_reconstructions_are_left_unchanged(sp, shape):
+    """spx_batch_set_active: idle slots keep their iterate while the active
+    ones advance exactly as they do alone (row-major, tiled and 3-D)."""
+    pats = patterns(shape, 3, seed=90)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    before = b.model()
+    b.set_active([1, 0, 1])
+    b.iterate(7)
+    got = b.model()
+    assert np.array_equal(got[1], before[1])
+    for r in (0, 2):
+        m, s, e = run_single(sp, pats[r], "hio", 7)
+        assert np.array_equal(got[r], m), r
+    # reactivated, the idle one continues from where it stopped
+    b.set_active(None)
+    b.iterate(3)
+    m, s, e = run_single(sp, pats[1], "hio", 3)
+    assert np.array_equal(b.model(1), m)
+    b.close()
