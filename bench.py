@@ -298,6 +298,41 @@ def volume_bench(dm_its=200, er_its=100, period=20):
             "support_fraction": frac, "plain_iterations": per_it}
 
 
+def reconstructor_bench(repeats=32, single_sample=4, seed=600):
+    """The Reconstructor front-end on a 256^2 disc pattern: RAAR (beta 0.9 ->
+    0.8) for 200 iterations then ER for 50, area shrinkwrap every 20, 5
+    image / score outputs.  reconstruct_loop(repeats) (all random starts in
+    one batch) against the reference's serial loop of reconstruct() calls,
+    timed on `single_sample` of them and extrapolated."""
+    import numpy as np
+    from libspimage_amd import workloads
+    from libspimage_amd.reconstructor import Reconstructor
+    p = workloads.disc_256(seed)
+    R = Reconstructor()
+    R.set_intensities(np.fft.fftshift(p.amplitudes.astype(np.float64) ** 2))
+    R.set_mask(np.fft.fftshift(p.mask.astype(bool)))
+    R.set_number_of_iterations(250)
+    R.set_number_of_outputs(5)
+    R.set_initial_support(radius=45)
+    R.set_support_algorithm("area", update_period=20, blur_init=3.0, blur_final=0.7, area_init=0.1, area_final=0.08)
+    R.append_phasing_algorithm("raar", number_of_iterations=200, beta_init=0.9, beta_final=0.8)
+    R.append_phasing_algorithm("er", number_of_iterations=50)
+    R.reconstruct(seed=1)  # warm-up (tuning, code objects)
+    t0 = time.perf_counter()
+    loop = R.reconstruct_loop(repeats, seed=seed)
+    t_loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    singles = [R.reconstruct(seed=seed + r) for r in range(single_sample)]
+    t_single = (time.perf_counter() - t0) / single_sample
+    same = all(float(loop["scores_final"]["fourier_error"][r]) == float(singles[r]["fourier_error"][-1])
+               for r in range(single_sample))
+    return {"workload": f"Reconstructor, 256x256 disc pattern, {repeats} random starts, RAAR 200 + ER 50, "
+                        "area shrinkwrap every 20, 5 outputs",
+            "reconstruct_loop_seconds": t_loop, "loop_recons_per_s": repeats / t_loop,
+            "serial_reconstruct_seconds_each": t_single, "serial_recons_per_s": 1.0 / t_single,
+            "speedup": t_single * repeats / t_loop, "final_errors_equal_in_sample": same}
+
+
 def _serving_pattern(args):
     from libspimage_amd import workloads
     shape, seed = args
@@ -430,6 +465,7 @@ def main():
     batched = None if args.no_batched else batched_bench()
     serving = None if args.no_serving else serving_bench()
     volume = None if args.no_volume else volume_bench()
+    reconstructor = None if args.no_batched else reconstructor_bench()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
@@ -452,6 +488,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "volume_256": volume,
+        "reconstructor": reconstructor,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -71,6 +71,11 @@ spimage_EXPORT int spx_batch_support_fraction(SpxBatch *b, float *fraction);
 /* results of reconstruction `rec` (rec < 0: all) */
 spimage_EXPORT int spx_batch_get_model(SpxBatch *b, int rec, float *model_complex);
 spimage_EXPORT int spx_batch_get_support(SpxBatch *b, int rec, int *support);
+/* the model before projection of the last iteration (sp_phaser_model_before_projection,
+ * reference src/phasing.c:496-513) and the Fourier transform of the current
+ * model (sp_phaser_fmodel, src/phasing.c:450-472), complex, row-major */
+spimage_EXPORT int spx_batch_get_model_before_projection(SpxBatch *b, int rec, float *model_complex);
+spimage_EXPORT int spx_batch_get_fmodel(SpxBatch *b, int rec, float *fmodel_complex);
 
 /* device time of `iterations` iterations (no support updates), HIP events */
 spimage_EXPORT int spx_batch_time_iterations(SpxBatch *b, int iterations, float *ms);

@@ -276,6 +276,22 @@ int spx_batch_get_model(SpxBatch* b, int rec, float* out) {
   return b->e->download(b->e->g1(), (float2*)out, rec);
 }
 
+int spx_batch_get_model_before_projection(SpxBatch* b, int rec, float* out) {
+  if (!check(b, rec) || !out) return fail(-1, "bad batch arguments");
+  return b->e->download(b->e->gp_buf(), (float2*)out, rec);
+}
+
+int spx_batch_get_fmodel(SpxBatch* b, int rec, float* out) {
+  if (!check(b, rec) || !out) return fail(-1, "bad batch arguments");
+  float2* dev = nullptr;
+  if (hipMalloc(&dev, sizeof(float2) * (size_t)b->N * b->nrec) != hipSuccess)
+    return fail(-20, "device memory for the Fourier model");
+  int rc = b->e->fmodel(dev);
+  if (!rc) rc = b->e->download(dev, (float2*)out, rec);
+  (void)hipFree(dev);
+  return rc;
+}
+
 int spx_batch_get_support(SpxBatch* b, int rec, int* out) {
   if (!check(b, rec) || !out) return fail(-1, "bad batch arguments");
   const long n = rec < 0 ? b->N * b->nrec : b->N;

@@ -283,6 +283,8 @@ _bind("spx_batch_time_iterations", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_synchronize", C.c_int, PBatch)
 _bind("spx_batch_init_model_rec", C.c_int, PBatch, C.c_int, C.c_int, C.c_ulonglong)
 _bind("spx_batch_set_active", C.c_int, PBatch, C.POINTER(C.c_int))
+_bind("spx_batch_get_model_before_projection", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_get_fmodel", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_query", C.c_int, PBatch)
 _bind("spx_batch_estimate_bytes", C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_device_memory", C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
@@ -391,6 +393,18 @@ class Batch:
         out = np.empty((n,) + self.shape, np.complex64)
         self._check(spx_batch_get_model(self.h, rec, out.ctypes.data_as(PF)), "get_model")
         return out if rec < 0 else out[0]
+
+    def _complex(self, fn, rec, what):
+        n = self.nrec if rec < 0 else 1
+        out = np.empty((n,) + self.shape, np.complex64)
+        self._check(fn(self.h, rec, out.ctypes.data_as(PF)), what)
+        return out if rec < 0 else out[0]
+
+    def model_before_projection(self, rec=-1):
+        return self._complex(spx_batch_get_model_before_projection, rec, "get_model_before_projection")
+
+    def fmodel(self, rec=-1):
+        return self._complex(spx_batch_get_fmodel, rec, "get_fmodel")
 
     def support(self, rec=-1):
         n = self.nrec if rec < 0 else 1
