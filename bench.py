@@ -13,10 +13,13 @@ Prints one JSON line with the roofline of the dominant pass (algorithmic
 bytes from the 78 N bytes of a 2-D HIO iteration: 41 N for the real-space
 pass, 37 N for the Fourier-space pass), the whole-iteration roofline, the
 1024^2 RAAR + shrinkwrap reconstruction (300 iterations, area support every
-20) and a CPU baseline: the oracle's restatement of the reference loop on one
-host core.
+20), the batched workload (128 patterns x 8 starts), the served workload
+(Poisson 200 req/s), the 256^3 DM + ER reconstruction, the Reconstructor
+front-end's reconstruct_loop, and a CPU baseline: the oracle's restatement of
+the reference loop on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-batched] [--no-serving] [--no-volume]
 """
 from __future__ import annotations
 
