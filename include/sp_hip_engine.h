@@ -31,9 +31,9 @@ spimage_EXPORT const char *spx_last_error(void);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
 /* Elements per thread (4, 8 or 16) of the transform variant the tuner chose
- * for lines of length L in row passes and in column passes, for 2-D shapes
- * whose Fourier-space arrays are row-major (tiled = 0) or in the 4-column
- * tiled layout (tiled = 1). */
+ * for lines of length L in row passes and in column passes.  One choice per
+ * length serves both layouts of the Fourier-space arrays; `tiled` is kept
+ * for callers that ask per layout. */
 spimage_EXPORT int spx_fft_variant(int L, int tiled, int *row_elements, int *col_elements);
 
 /* 1 when the phaser's engine keeps its Fourier-space work arrays (the

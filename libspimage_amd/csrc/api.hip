@@ -992,8 +992,9 @@ int spx_fft_variant(int L, int tiled, int* row_e, int* col_e) {
     spx::set_error("length %d is not supported", L);
     return -11;
   }
-  if (row_e) *row_e = e->var[tiled ? e->row_t4_choice : e->row_choice].E;
-  if (col_e) *col_e = e->var[tiled ? e->col_t4_choice : e->col_choice].E;
+  (void)tiled;  // one choice per length serves both layouts
+  if (row_e) *row_e = e->var[e->row_choice].E;
+  if (col_e) *col_e = e->var[e->col_choice].E;
   return 0;
 }
 

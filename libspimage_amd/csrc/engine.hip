@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -112,7 +113,7 @@ __global__ void k_tune_fill(float2* w, float2* g0, float2* g1, float* amp, uint8
 }
 
 static std::mutex tune_mu;
-// per length: bit 1 row tuned, 2 column tuned, 4 tiled row tuned, 8 tiled column tuned
+// per length: bit 1 row variant chosen, bit 2 column variant chosen
 static std::map<LengthEntry*, int> tune_done;
 
 // variant with E elements per thread (forced by SPX_FFT_E, else 8) among
@@ -204,120 +205,121 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
 // iteration runs them: column pass then row pass, in sequence, because a
 // variant's cost depends on where (which XCD's cache) the previous pass left
 // the data.  Both layouts of the Fourier-space arrays are timed (row-major,
-// and 4-column tiled when nx % 4 == 0 and a tiled row kernel exists); the
-// choice of variants is kept per length and layout, the choice of layout
-// per length pair.  Returns true for the tiled layout.
-static std::map<std::pair<LengthEntry*, LengthEntry*>, bool> tune_layout;
+// and 4-column tiled when nx % 4 == 0 and the row variant has a tiled
+// kernel).  The variants are chosen once per length and serve both layouts,
+// so that a reconstruction computes the same bits alone and in a batch
+// whatever layout each uses; the layout is chosen per length pair, for
+// single reconstructions and batches apart (in a batch whole reconstructions
+// stay on one XCD, which favours the tiled layout).  Returns true for the
+// tiled layout.
+static std::map<std::tuple<LengthEntry*, LengthEntry*, bool>, bool> tune_layout;
 
-static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_possible) {
+static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_shape) {
   std::lock_guard<std::mutex> lock(tune_mu);
-  // SPX_NO_T4=1 / SPX_T4=1 force the layout (the tuned choice is kept)
+  const bool need_row = !(tune_done[ex] & 1), need_col = !(tune_done[ey] & 2);
+  tune_done[ex] |= 1;
+  tune_done[ey] |= 2;
+  if (need_row) ex->row_choice = default_choice(ex, false);
+  if (need_col) ey->col_choice = default_choice(ey, false);
+  // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows it)
   const char* not4 = getenv("SPX_NO_T4");
   const char* yes4 = getenv("SPX_T4");
-  auto forced = [&](bool t4) {
+  auto decided = [&](bool t4) {
+    const bool possible = t4_shape && ex->var[ex->row_choice].t4;
     if (not4 && atoi(not4) == 1) return false;
-    if (yes4 && atoi(yes4) == 1) return t4_possible;
-    return t4 && t4_possible;
+    if (yes4 && atoi(yes4) == 1) return possible;
+    return t4 && possible;
   };
-  const auto key = std::make_pair(ex, ey);
+  const auto key = std::make_tuple(ex, ey, nrec > 1);
   auto it = tune_layout.find(key);
-  if (it != tune_layout.end()) return forced(it->second);
-  float best_t[2] = {1e30f, 1e30f};
-  for (int lay = 0; lay < 2; ++lay) {
+  if (it != tune_layout.end()) return decided(it->second);
+  if (tuning_off()) {
+    tune_layout[key] = false;
+    return decided(false);
+  }
+  const int nx = ex->L;
+  const long N = (long)nx * ny;
+  // cap the scratch at 16 Mi elements (whole reconstructions)
+  int nr = nrec;
+  if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
+  if (nr < 1) nr = 1;
+  const long n = (long)nr * N;
+  hipStream_t s = nullptr;
+  float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+  float* amp = nullptr;
+  uint8_t* fl = nullptr;
+  bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
+            hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+            hipMalloc(&fl, (size_t)n) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  float best = 1e30f;
+  int br = ex->row_choice, bc = ey->col_choice, blay = 0;
+  for (int lay = 0; ok && lay < 2; ++lay) {
     const bool t4 = lay == 1;
-    if (t4 && !t4_possible) continue;
-    const int rbit = t4 ? 4 : 1, cbit = t4 ? 8 : 2;
-    const bool need_row = !(tune_done[ex] & rbit), need_col = !(tune_done[ey] & cbit);
-    tune_done[ex] |= rbit;
-    tune_done[ey] |= cbit;
-    int& rch = t4 ? ex->row_t4_choice : ex->row_choice;
-    int& cch = t4 ? ey->col_t4_choice : ey->col_choice;
-    if (need_row) rch = default_choice(ex, t4);
-    if (need_col) cch = default_choice(ey, false);
-    if (tuning_off()) continue;
-    const int nx = ex->L;
-    const long N = (long)nx * ny;
-    // cap the scratch at 16 Mi elements (whole reconstructions)
-    int nr = nrec;
-    if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
-    if (nr < 1) nr = 1;
+    if (t4 && !t4_shape) continue;
     LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
     if (t4) {
       g.tshift = 2;
       g.tstride = 4L * ny;
     }
-    const long n = (long)nr * N;
-    hipStream_t s = nullptr;
-    float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
-    float* amp = nullptr;
-    uint8_t* fl = nullptr;
-    bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
-              hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
-              hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
-              hipMalloc(&fl, (size_t)n) == hipSuccess;
-    if (ok) {
-      hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
-      ok = hipGetLastError() == hipSuccess;
-    }
-    if (ok) {
-      int br = rch, bc = cch;
-      for (int r = 0; r < ex->nvar; ++r) {
-        if ((!need_row && r != rch) || (t4 && !ex->var[r].t4)) continue;
-        for (int c = 0; c < ey->nvar; ++c) {
-          if (!need_col && c != cch) continue;
-          ColArgs ca{};
-          ca.in = w;
-          ca.W = w;
-          ca.geo = g;
-          ca.ampx = amp;
-          ca.nt = 1;
-          UpdateArgs u{};
-          u.W = w;
-          u.gcur = g0;
-          u.gnew = g1;
-          u.flags = fl;
-          u.nlines = nr * ny;
-          u.lines_per_rec = ny;
-          u.alg = kHIO;
-          u.beta = 0.9f;
-          u.nt = 1;
-          u.scale = 1.f;
-          u.nsize = 1.f;
-          const Variant &vr = ex->var[r], &vc = ey->var[c];
-          // steady state, iterates ping-ponging as in a run
-          float2 *ga = g0, *gb = g1;
-          const float ms = time_launches(
-              s,
-              [&] {
-                u.gcur = ga;
-                u.gnew = gb;
-                vc.col(kColProject, ca, s);
-                vr.update(u, false, t4, s);
-                std::swap(ga, gb);
-              },
-              20, 30);
-          if (getenv("SPX_TUNE_LOG"))
-            fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr, t4 ? "tiled" : "row-major",
-                    vr.E, vc.E, ms * 1000.f);
-          if (ms < best_t[lay]) {
-            best_t[lay] = ms;
-            br = r;
-            bc = c;
-          }
+    for (int r = 0; r < ex->nvar; ++r) {
+      if ((!need_row && r != ex->row_choice) || (t4 && !ex->var[r].t4)) continue;
+      for (int c = 0; c < ey->nvar; ++c) {
+        if (!need_col && c != ey->col_choice) continue;
+        ColArgs ca{};
+        ca.in = w;
+        ca.W = w;
+        ca.geo = g;
+        ca.ampx = amp;
+        ca.nt = 1;
+        UpdateArgs u{};
+        u.W = w;
+        u.flags = fl;
+        u.nlines = nr * ny;
+        u.lines_per_rec = ny;
+        u.alg = kHIO;
+        u.beta = 0.9f;
+        u.scale = 1.f;
+        u.nsize = 1.f;
+        u.nt = 1;
+        const Variant &vr = ex->var[r], &vc = ey->var[c];
+        // steady state, iterates ping-ponging as in a run
+        float2 *ga = g0, *gb = g1;
+        const float ms = time_launches(
+            s,
+            [&] {
+              u.gcur = ga;
+              u.gnew = gb;
+              vc.col(kColProject, ca, s);
+              vr.update(u, false, t4, s);
+              std::swap(ga, gb);
+            },
+            20, 30);
+        if (getenv("SPX_TUNE_LOG"))
+          fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr, t4 ? "tiled" : "row-major",
+                  vr.E, vc.E, ms * 1000.f);
+        if (ms < best) {
+          best = ms;
+          br = r;
+          bc = c;
+          blay = lay;
         }
       }
-      rch = br;
-      cch = bc;
     }
-    (void)hipStreamSynchronize(s);
-    void* bufs[] = {w, g0, g1, amp, fl};
-    for (void* p : bufs)
-      if (p) (void)hipFree(p);
-    if (s) (void)hipStreamDestroy(s);
   }
-  const bool t4 = t4_possible && best_t[1] < best_t[0];
-  tune_layout[key] = t4;
-  return forced(t4);
+  ex->row_choice = br;
+  ey->col_choice = bc;
+  (void)hipStreamSynchronize(s);
+  void* bufs[] = {w, g0, g1, amp, fl};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (s) (void)hipStreamDestroy(s);
+  tune_layout[key] = blay == 1;
+  return decided(blay == 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,8 +659,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt.tshift = 2;
   sh->gyt.tstride = 4L * ny;
   if (found[0] && found[1] && !found[2]) {
-    const bool t4_possible = nx % 4 == 0 && found[0]->has_t4();
-    sh->t4 = tune_pair(found[0], found[1], ny, nrec, t4_possible);
+    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0);
   } else {
     if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
     if (found[1]) tune_length(found[1], 0, &sh->gy);
@@ -746,7 +747,7 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     a.W = out;
     a.geo = tiled ? sh.gyt : sh.gy;
     a.outer_per_rec = 1;
-    sh.ey->col(mode, a, s, tiled);
+    sh.ey->col(mode, a, s);
   }
 }
 
@@ -759,7 +760,7 @@ static int last_axis_tiles_per_rec(const Shape& sh) {
   }
   LineGeom g = sh.t4 ? sh.gyt : sh.gy;
   g.nouter = 1;
-  return sh.ey->col_blocks(g, sh.t4);
+  return sh.ey->col_blocks(g);
 }
 
 // ---------------------------------------------------------------------------

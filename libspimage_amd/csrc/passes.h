@@ -749,33 +749,20 @@ constexpr Variant variant() {
 }
 
 // Dispatch table entry for one length: its compiled variants and the ones in
-// use (chosen by the tuner) for row and column passes, separately for the
-// natural layout and for 2-D shapes in the 4-column tiled layout.
+// use (chosen by the tuner) for row and column passes, in either layout.
 struct LengthEntry {
   int L;
   int nvar;
   Variant var[3];
   int row_choice;
   int col_choice;
-  int row_t4_choice;
-  int col_t4_choice;
-  bool has_t4() const {
-    for (int i = 0; i < nvar; ++i)
-      if (var[i].t4) return true;
-    return false;
-  }
   void row(int mode, const RowArgs& a, hipStream_t s) const { var[row_choice].row(mode, a, s); }
   void update(const UpdateArgs& a, bool dm, bool tiled, hipStream_t s) const {
-    if (tiled) var[row_t4_choice].update(a, dm, true, s);
-    else var[row_choice].update(a, dm, false, s);
+    var[row_choice].update(a, dm, tiled, s);
   }
-  void row_t4(const RowArgs& a, hipStream_t s) const { var[row_t4_choice].row_t4(a, s); }
-  void col(int mode, const ColArgs& a, hipStream_t s, bool tiled = false) const {
-    var[tiled ? col_t4_choice : col_choice].col(mode, a, s);
-  }
-  int col_blocks(const LineGeom& g, bool tiled = false) const {
-    return var[tiled ? col_t4_choice : col_choice].col_blocks(g);
-  }
+  void row_t4(const RowArgs& a, hipStream_t s) const { var[row_choice].row_t4(a, s); }
+  void col(int mode, const ColArgs& a, hipStream_t s) const { var[col_choice].col(mode, a, s); }
+  int col_blocks(const LineGeom& g) const { return var[col_choice].col_blocks(g); }
 };
 
 // Registry implemented by the instantiation units (mutable: the tuner sets
