@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "engine.h"
@@ -808,9 +809,69 @@ SpSupportAlgorithm* sp_support_centre_image_alloc(void) {
   return r;
 }
 
-SpSupportAlgorithm* sp_support_template_alloc(Image*, real, sp_smap*) {
-  fprintf(stderr, "sp_support_template_alloc: the template support update is not provided by the HIP engine\n");
-  return nullptr;
+namespace {
+// sp_cabs (include/spimage/linear_alg.h:163-165): the sum in float, then the
+// square root rounded to float
+inline float cabs_host(Complex c) { return sqrtf(sp_imag(c) * sp_imag(c) + sp_real(c) * sp_real(c)); }
+
+// The template parameters as callers see them (SpSupportTemplateParameters
+// first), then this library's own: a key naming the device copy of the
+// template and the template's magnitudes |blured| on the host.
+struct TemplateParams {
+  SpSupportTemplateParameters base;
+  unsigned long long key;
+  float* abs_blured;
+};
+unsigned long long g_template_key = 0;
+
+long template_index(long n, real area, real fraction) {
+  // (int)(size * area * fraction) in float (src/support_update.c:72,195);
+  // clamped to the array
+  long k = (long)(int)((real)n * area * fraction);
+  if (k >= n) k = n - 1;
+  if (k < 0) k = 0;
+  return k;
+}
+}  // namespace
+
+// reference src/support_update.c:47-85
+SpSupportAlgorithm* sp_support_template_alloc(Image* initial_support, real blur_radius, sp_smap* area) {
+  if (!initial_support || !area) return nullptr;
+  Image* blured = sp_gaussian_blur(initial_support, blur_radius);
+  if (!blured) return nullptr;
+  TemplateParams* x = (TemplateParams*)calloc(1, sizeof(TemplateParams));
+  SpSupportTemplateParameters* p = &x->base;
+  p->blured = blured;
+  p->area = area;
+  const long n = sp_image_size(initial_support);
+  p->original_area = 0;
+  for (long i = 0; i < n; i++)
+    if (sp_real(initial_support->image->data[i]) != 0 || sp_imag(initial_support->image->data[i]) != 0)
+      p->original_area += 1.0;
+  p->original_area /= (real)n;
+  // the blurred template sorted by descending magnitude; equal magnitudes
+  // may come in any order, as with the reference's qsort
+  x->abs_blured = (float*)malloc(sizeof(float) * n);
+  std::vector<long> order((size_t)n);
+  for (long i = 0; i < n; i++) {
+    x->abs_blured[i] = cabs_host(blured->image->data[i]);
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](long a, long b) { return x->abs_blured[a] > x->abs_blured[b]; });
+  p->sorted = sp_image_duplicate(blured, SP_COPY_DATA);
+  for (long i = 0; i < n; i++) p->sorted->image->data[i] = blured->image->data[order[i]];
+  const real max_threshold =
+      cabs_host(p->sorted->image->data[template_index(n, spx_smap_max(area), p->original_area)]);
+  const real min_threshold =
+      cabs_host(p->sorted->image->data[template_index(n, spx_smap_min(area), p->original_area)]);
+  if (max_threshold < 1e-4) fprintf(stderr, "Dangerously large template area with this threshold!\n");
+  if (1.0 - min_threshold < 1e-4) fprintf(stderr, "Dangerously small template area with this threshold!\n");
+  x->key = __atomic_add_fetch(&g_template_key, 1, __ATOMIC_RELAXED);
+  SpSupportAlgorithm* r = (SpSupportAlgorithm*)malloc(sizeof(SpSupportAlgorithm));
+  r->type = SpSupportTemplate;
+  r->params = x;
+  r->function = (void*)sp_support_template_update_support;
+  return r;
 }
 
 SpSupportArray* sp_support_array_alloc(int size, int update_period) {
@@ -871,19 +932,141 @@ int sp_support_threshold_update_support(SpSupportAlgorithm* alg, SpPhaser* ph) {
 
 int sp_support_static_update_support(SpSupportAlgorithm*, SpPhaser*) { return 0; }
 
-int sp_support_template_update_support(SpSupportAlgorithm*, SpPhaser*) {
-  fprintf(stderr, "sp_support_template_update_support: not provided by the HIP engine\n");
-  return -1;
+// reference src/support_update.c:181-207: the support is the blurred
+// template thresholded at the magnitude of its sorted entry
+// (int)(N * area(iteration) * original_area)
+int sp_support_template_update_support(SpSupportAlgorithm* alg, SpPhaser* ph) {
+  TemplateParams* x = (TemplateParams*)alg->params;
+  SpSupportTemplateParameters* p = &x->base;
+  Engine* e = engine_of(ph);
+  if (!e) return -1;
+  const long n = sp_image_size(p->sorted);
+  if (n != ph->image_size) {
+    fprintf(stderr, "sp_support_template_update_support: the template has %ld pixels, the phaser %d\n", n,
+            ph->image_size);
+    return -1;
+  }
+  const real area = spx_smap_interpolate(p->area, (real)ph->iteration);
+  const real t = cabs_host(p->sorted->image->data[template_index(n, area, p->original_area)]);
+  if (e->support_template(x->key, x->abs_blured, t)) {
+    report("sp_support_template_update_support");
+    return -1;
+  }
+  hs(ph)->device_flags_newer = true;
+  return 0;
 }
 
-int sp_support_close_update_support(SpSupportAlgorithm*, SpPhaser*) {
-  fprintf(stderr, "sp_support_close_update_support: not provided by the HIP engine\n");
-  return -1;
+// reference src/support_update.c:213-280, on the device
+int sp_support_close_update_support(SpSupportAlgorithm* alg, SpPhaser* ph) {
+  SpSupportCloseParameters* p = (SpSupportCloseParameters*)alg->params;
+  Engine* e = engine_of(ph);
+  if (!e) return -1;
+  if (e->support_close(p->size)) {
+    report("sp_support_close_update_support");
+    return -1;
+  }
+  hs(ph)->device_flags_newer = true;
+  return 0;
 }
 
-int sp_support_centre_image(SpSupportAlgorithm*, SpPhaser*) {
-  fprintf(stderr, "sp_support_centre_image: not provided by the HIP engine\n");
-  return -1;
+// reference src/support_update.c:282-389 (sp_support_centre_image_cpu) with
+// sp_pixel_flags_translate_mask (src/image_util.c:3587-3611) and
+// sp_i3matrix_binary_center_of_mass_shifted (src/linear_alg.c:444-475).
+//
+// The support bit moves cyclically; the other flag bits stay in place.  The
+// reference also translates its cached copy of the model (ph->model), which
+// the next model read replaces because the update consumes an iteration;
+// the iterate itself does not move, so it does not move here either.
+//
+// The translations compose, so they are tracked as one offset o (support at
+// p = original support at p - o) and applied once.  The centre of mass is
+// summed in float in raster order, as the reference sums it, which is why
+// this update runs on the host: its result depends on that order once the
+// sums pass 2^24.  It costs one download and one upload of the N flag bytes.
+int sp_support_centre_image(SpSupportAlgorithm*, SpPhaser* ph) {
+  Engine* e = engine_of(ph);
+  if (!e) return -1;
+  const int nx = ph->nx, ny = ph->ny, nz = ph->nz;
+  const long n = ph->image_size;
+  std::vector<uint8_t> f((size_t)n);
+  if (e->download_flags(f.data())) {
+    report("sp_support_centre_image");
+    return -1;
+  }
+  int ox = 0, oy = 0, oz = 0;
+  auto wrap = [](long v, int m) { return (int)(((v % m) + m) % m); };
+  auto inside = [&](int x, int y, int z) {
+    const long i = ((long)wrap(z - oz, nz) * ny + wrap(y - oy, ny)) * nx + wrap(x - ox, nx);
+    return (f[i] & SpPixelInsideSupport) != 0;
+  };
+  auto translate = [&](int dx, int dy, int dz) {
+    ox = wrap((long)ox + dx, nx);
+    oy = wrap((long)oy + dy, ny);
+    oz = wrap((long)oz + dz, nz);
+  };
+  if (nz == 1) {  // SP_2D (the reference's 1-2 dimension branch)
+    for (int ix = 0; ix < nx; ix++)
+      if (inside(ix, (ny - 1) / 2, 0)) {
+        translate(0, (ny - 1) / 2, 0);
+        break;
+      }
+    for (int iy = 0; iy < ny; iy++)
+      if (inside((nx - 1) / 2, iy, 0)) {
+        translate((nx - 1) / 2, 0, 0);
+        break;
+      }
+  } else {
+    // each break leaves the inner loop only, so one scan may translate
+    // more than once, as in the reference
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++)
+        if (inside((nx - 1) / 2, iy, iz)) {
+          translate((nx - 1) / 2, 0, 0);
+          break;
+        }
+    for (int ix = 0; ix < nx; ix++)
+      for (int iz = 0; iz < nz; iz++)
+        if (inside(ix, (ny - 1) / 2, iz)) {
+          translate(0, (ny - 1) / 2, 0);
+          break;
+        }
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++)
+        if (inside(ix, iy, (nz - 1) / 2)) {
+          translate(0, 0, (nz - 1) / 2);
+          break;
+        }
+  }
+  float c[3] = {0.f, 0.f, 0.f};
+  int count = 0;
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++)
+        if (inside(x, y, z)) {
+          c[0] += (float)((x + (nx - 1) / 2) % nx);
+          c[1] += (float)((y + (ny - 1) / 2) % ny);
+          c[2] += (float)((z + (nz - 1) / 2) % nz);
+          count++;
+        }
+  if (count) {
+    for (int k = 0; k < 3; k++) c[k] /= (float)count;
+    c[0] -= (float)((nx - 1) / 2);
+    c[1] -= (float)((ny - 1) / 2);
+    c[2] -= (float)((nz - 1) / 2);
+  }
+  translate((int)(0.f - c[0]), (int)(0.f - c[1]), (int)(0.f - c[2]));
+  std::vector<uint8_t> out((size_t)n);
+  long i = 0;
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++, i++)
+        out[i] = (uint8_t)((f[i] & ~SpPixelInsideSupport) | (inside(x, y, z) ? SpPixelInsideSupport : 0));
+  if (e->upload_flags(out.data())) {
+    report("sp_support_centre_image");
+    return -1;
+  }
+  hs(ph)->device_flags_newer = true;
+  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <vector>
+#include <utility>
 
 #include "passes.h"
 
@@ -127,6 +128,13 @@ class Engine {
   int shrinkwrap_area(float sigma, float area);
   int shrinkwrap_threshold(float sigma, float rel);
   int patterson_support(const float* amp_raw, float value);
+  // closing of the support: `size` dilations, then `size` erosions, with the
+  // 6-neighbour cross (reference src/support_update.c:213-280)
+  int support_close(int size);
+  // support = tmpl_abs > t (reference src/support_update.c:181-207).  The N
+  // host values of tmpl_abs are uploaded on the first call with `key` and
+  // kept on the device for the engine's lifetime.
+  int support_template(unsigned long long key, const float* tmpl_abs, float t);
   // threshold used by the last area/threshold/Patterson update (per rec)
   int last_threshold(float* t);
 
@@ -166,6 +174,7 @@ class Engine {
   int* alg_rec = nullptr;     // optional per-rec tables
   float* beta_rec = nullptr;
   uint8_t* rec_active = nullptr;
+  std::vector<std::pair<unsigned long long, float*>> templates;  // support templates by key
   bool some_idle = false;
   bool R_valid = false, Rf_valid = false;
   std::vector<int> R_stale;  // reconstructions whose W is out of date

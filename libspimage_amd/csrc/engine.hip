@@ -602,6 +602,41 @@ __global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl_
   }
 }
 
+// One pass of the support closing (reference src/support_update.c:213-280):
+// a dilation (dilate = 1: inside if the pixel or one of its 6 neighbours is)
+// or an erosion (outside if the pixel or one of its neighbours is).
+// Neighbours beyond the array's faces do not take part, and reconstructions
+// stacked along z do not see each other.  The other flag bits are copied.
+__global__ void k_close_pass(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int nx, int ny, int nz,
+                             long N, long n, int dilate) {
+  const long sy = nx, sz = (long)nx * ny;
+  GRID_STRIDE(i, n) {
+    const long j = i % N;
+    const int x = (int)(j % nx);
+    const long r = j / nx;
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    const bool want = dilate != 0;  // the neighbour state that flips the pixel
+    bool flip = ((src[i] & kInsideSupport) != 0) == want;
+    if (x > 0) flip |= ((src[i - 1] & kInsideSupport) != 0) == want;
+    if (x < nx - 1) flip |= ((src[i + 1] & kInsideSupport) != 0) == want;
+    if (y > 0) flip |= ((src[i - sy] & kInsideSupport) != 0) == want;
+    if (y < ny - 1) flip |= ((src[i + sy] & kInsideSupport) != 0) == want;
+    if (z > 0) flip |= ((src[i - sz] & kInsideSupport) != 0) == want;
+    if (z < nz - 1) flip |= ((src[i + sz] & kInsideSupport) != 0) == want;
+    const bool inside = want ? flip : !flip;
+    dst[i] = (uint8_t)((src[i] & ~kInsideSupport) | (inside ? kInsideSupport : 0));
+  }
+}
+
+// support = tmpl[i] > t for every reconstruction (tmpl holds one of N)
+// (reference src/support_update.c:196-200 with :392-400)
+__global__ void k_template_mask(const float* __restrict__ tmpl, uint8_t* flags, long N, long n, float t) {
+  GRID_STRIDE(i, n) {
+    const uint8_t f = flags[i];
+    flags[i] = (tmpl[i % N] > t) ? (uint8_t)(f | kInsideSupport) : (uint8_t)(f & ~kInsideSupport);
+  }
+}
+
 inline dim3 ew_grid(long n) {
   long b = (n + 255) / 256;
   if (b > 8192) b = 8192;
@@ -831,6 +866,7 @@ Engine::~Engine() {
                   phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto& e : templates) (void)hipFree(e.second);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (s) (void)hipStreamDestroy(s);
@@ -1308,6 +1344,37 @@ int Engine::shrinkwrap_threshold(float sigma, float rel) {
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, ctl, nrec, 0u);
   if (int rc = blur_to_abs(sigma, false)) return rc;
   hipLaunchKernelGGL(k_mask, dim3(rec_blocks(N), nrec), dim3(256), 0, s, absb, flags, N, ctl, 1, rel);
+  SPX_HIP(hipGetLastError());
+  ++flags_version;
+  return 0;
+}
+
+// The passes ping-pong between the flags and the byte view of `absb`
+// (4 NT bytes); 2 size passes end back in the flags.
+int Engine::support_close(int size) {
+  if (size <= 0) return 0;
+  uint8_t* tmp = reinterpret_cast<uint8_t*>(absb);
+  uint8_t *src = flags, *dst = tmp;
+  for (int k = 0; k < 2 * size; ++k) {
+    hipLaunchKernelGGL(k_close_pass, ew_grid(NT), dim3(256), 0, s, src, dst, nx, ny, nz, N, NT, k < size ? 1 : 0);
+    std::swap(src, dst);
+  }
+  SPX_HIP(hipGetLastError());
+  ++flags_version;
+  return 0;
+}
+
+int Engine::support_template(unsigned long long key, const float* tmpl_abs, float t) {
+  float* d = nullptr;
+  for (auto& e : templates)
+    if (e.first == key) d = e.second;
+  if (!d) {
+    SPX_HIP(hipMalloc(&d, sizeof(float) * N));
+    templates.emplace_back(key, d);
+    SPX_HIP(hipMemcpyAsync(d, tmpl_abs, sizeof(float) * N, hipMemcpyHostToDevice, s));
+    SPX_HIP(hipStreamSynchronize(s));
+  }
+  hipLaunchKernelGGL(k_template_mask, ew_grid(NT), dim3(256), 0, s, d, flags, N, NT, t);
   SPX_HIP(hipGetLastError());
   ++flags_version;
   return 0;

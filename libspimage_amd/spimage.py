@@ -220,6 +220,7 @@ _bind("sp_support_area_alloc", PSupAlg, PSmap, PSmap)
 _bind("sp_support_static_alloc", PSupAlg)
 _bind("sp_support_close_alloc", PSupAlg, C.c_int)
 _bind("sp_support_centre_image_alloc", PSupAlg)
+_bind("sp_support_template_alloc", PSupAlg, PImage, C.c_float, PSmap)
 _bind("sp_support_array_alloc", PSupArr, C.c_int, C.c_int)
 _bind("sp_support_array_init", PSupArr, PSupAlg, C.c_int)
 _bind("sp_support_array_set", None, PSupArr, C.c_int, PSupAlg)

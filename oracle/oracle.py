@@ -22,7 +22,7 @@ HERE = Path(__file__).resolve().parent
 BUILD = HERE / "_build"
 
 HIO, RAAR, DIFFMAP, ER = 1, 2, 3, 4
-SUP_NONE, SUP_THRESHOLD, SUP_AREA, SUP_STATIC = 0, 1, 2, 5
+SUP_NONE, SUP_THRESHOLD, SUP_AREA, SUP_CENTRE, SUP_TEMPLATE, SUP_STATIC, SUP_CLOSE = 0, 1, 2, 3, 4, 5, 6
 
 
 def build(force: bool = False) -> None:
@@ -40,7 +40,8 @@ def _define_structs(R):
     OrSup_ = type("OrSup", (C.Structure,), {"_fields_": [
         ("type", C.c_int), ("period", C.c_int), ("nblur", C.c_int),
         ("blur_keys", R * 32), ("blur_vals", R * 32), ("nval", C.c_int),
-        ("val_keys", R * 32), ("val_vals", R * 32)]})
+        ("val_keys", R * 32), ("val_vals", R * 32), ("close_size", C.c_int),
+        ("tmpl_abs", C.c_void_p), ("tmpl_sorted", C.c_void_p), ("tmpl_area", R)]})
     return OrAlg_, OrSup_
 
 
@@ -73,6 +74,12 @@ class Oracle:
             getattr(L, name).argtypes = [P]
         L.or_iterate.restype = C.c_int
         L.or_iterate.argtypes = [P, C.POINTER(self.OrAlg), C.POINTER(self.OrSup), C.c_int]
+        L.or_iterate_multi.restype = C.c_int
+        L.or_iterate_multi.argtypes = [P, C.POINTER(self.OrAlg), C.POINTER(self.OrSup), C.c_int, C.c_int]
+        L.or_template_prepare.argtypes = [P, C.c_int, C.c_int, C.c_int, R, P, P, C.POINTER(R)]
+        L.or_support_template_update.argtypes = [P, P, P, R, R]
+        L.or_support_close_update.argtypes = [P, C.c_int]
+        L.or_support_centre_image.argtypes = [P]
         L.or_efourier.restype = R
         L.or_efourier.argtypes = [P, C.POINTER(R)]
         L.or_ereal.restype = R
@@ -142,10 +149,31 @@ class Oracle:
         a.gamma2 = gamma2
         return a
 
-    def sup(self, type_: int = SUP_NONE, period: int = 1, blur=None, values=None):
+    def template(self, tmpl: np.ndarray, radius: float):
+        """sp_support_template_alloc's set-up: (|blurred|, sorted descending, area fraction)."""
+        a = np.ascontiguousarray(tmpl, dtype=self.np_cplx)
+        nz, ny, nx = _zyx(a.shape)
+        absv = np.empty(a.shape, self.np_real)
+        srt = np.empty(a.size, self.np_real)
+        area = self.R()
+        self.lib.or_template_prepare(a.ctypes.data, nx, ny, nz, radius, absv.ctypes.data, srt.ctypes.data,
+                                     C.byref(area))
+        return absv, srt, float(area.value)
+
+    def sup(self, type_: int = SUP_NONE, period: int = 1, blur=None, values=None, close_size: int = 0,
+            template=None):
+        """One support algorithm; `template` is the tuple returned by template()."""
         s = self.OrSup()
         s.type = type_
         s.period = period
+        s.close_size = close_size
+        if template is not None:
+            absv, srt, area = (np.ascontiguousarray(template[0], self.np_real),
+                               np.ascontiguousarray(template[1], self.np_real), template[2])
+            s.tmpl_abs = absv.ctypes.data
+            s.tmpl_sorted = srt.ctypes.data
+            s.tmpl_area = area
+            s._keep = (absv, srt)  # the arrays must outlive the structure
         for name, pairs in (("blur", blur or [(0, 0.0)]), ("val", values or [(0, 0.0)])):
             setattr(s, "nblur" if name == "blur" else "nval", len(pairs))
             keys = getattr(s, f"{name}_keys")
@@ -263,6 +291,10 @@ class OraclePhaser:
         self.gp[...] = 0
 
     def iterate(self, alg, n: int, sup=None) -> int:
+        """`sup`: one support algorithm, or a list of them run in order at each update."""
+        if isinstance(sup, (list, tuple)):
+            arr = (self.o.OrSup * len(sup))(*sup)
+            return int(self.o.lib.or_iterate_multi(self.h, C.byref(alg), arr, len(sup), n))
         return int(self.o.lib.or_iterate(self.h, C.byref(alg), C.byref(sup) if sup is not None else None, n))
 
     def efourier(self):
@@ -289,3 +321,16 @@ class OraclePhaser:
     def support_threshold_update(self, radius: float, rel: float) -> float:
         self.o.lib.or_support_threshold_update(self.h, radius, rel)
         return float(self.o.lib.or_last_threshold(self.h))
+
+    def support_template_update(self, template, area: float) -> float:
+        """`template`: the tuple of Oracle.template()."""
+        absv = np.ascontiguousarray(template[0], self.o.np_real)
+        srt = np.ascontiguousarray(template[1], self.o.np_real)
+        self.o.lib.or_support_template_update(self.h, absv.ctypes.data, srt.ctypes.data, area, template[2])
+        return float(self.o.lib.or_last_threshold(self.h))
+
+    def support_close_update(self, size: int) -> None:
+        self.o.lib.or_support_close_update(self.h, size)
+
+    def support_centre_image(self) -> None:
+        self.o.lib.or_support_centre_image(self.h)

@@ -28,6 +28,12 @@
  *                        src/image_util.c:1012-1032
  *   or_support_area_update       src/support_update.c:136-151,392-400,443-454
  *   or_support_threshold_update  src/support_update.c:167-179
+ *   or_template_prepare, or_support_template_update
+ *                        src/support_update.c:47-85,181-207
+ *   or_support_close_update      src/support_update.c:213-280
+ *   or_support_centre_image      src/support_update.c:282-389 with
+ *                        src/image_util.c:3587-3611 and src/linear_alg.c:444-475
+ *   or_iterate_multi     src/phasing.c:995-1024 with src/support_update.c:489-494
  *   or_bezier            include/spimage/map.h:103-141 (solved here by
  *                        bisection on the curve, not by the closed form)
  *
@@ -39,7 +45,11 @@
  * tests/ImageTests.c:194-243); the area threshold against a brute-force
  * sort; the Bezier schedule against its end points, midpoint and the
  * engine's Cardano solution; the scheduler against the reference's integer
- * recurrence enumerated by hand.
+ * recurrence enumerated by hand; the closing against scipy.ndimage's binary
+ * dilation and erosion with the cross structure; the centre-image update
+ * against np.roll of the support by the shift that moves its float64
+ * centre of mass to the origin, and against hand-built cases of the
+ * half-image shift; the template threshold against a brute-force sort.
  */
 #include <math.h>
 #include <stdint.h>
@@ -248,12 +258,16 @@ typedef struct {
 } OrAlg;
 
 typedef struct {
-  int type; /* 0 none, 1 threshold, 2 area, 5 static */
+  int type; /* 0 none, 1 threshold, 2 area, 3 centre image, 4 template, 5 static, 6 close */
   int period;
   int nblur;
   R blur_keys[32], blur_vals[32];
   int nval;
   R val_keys[32], val_vals[32]; /* area or relative threshold */
+  int close_size;               /* close: dilations (= erosions) */
+  const R *tmpl_abs;            /* template: |blurred template|, N values */
+  const R *tmpl_sorted;         /* template: the same sorted descending */
+  R tmpl_area;                  /* template: original area fraction */
 } OrSup;
 
 typedef struct {
@@ -555,18 +569,176 @@ EXPORT void or_support_threshold_update(OrPhaser *p, R radius, R rel) {
   free(m);
 }
 
+/* |blur(template, radius)|, the same values sorted descending, and the
+ * template's area fraction: the set-up of sp_support_template_alloc
+ * (src/support_update.c:47-85).  The fraction counts pixels with a non-zero
+ * real or imaginary part, accumulated in R like the reference's. */
+EXPORT void or_template_prepare(const oc *tmpl, int nx, int ny, int nz, R radius, R *abs_out, R *sorted_out,
+                                R *area_out) {
+  const long N = (long)nx * ny * nz;
+  oc *bl = (oc *)malloc(sizeof(oc) * N);
+  or_gaussian_blur(tmpl, bl, nx, ny, nz, radius);
+  R area = 0;
+  for (long i = 0; i < N; ++i) {
+    abs_out[i] = cabs_ref(bl[i]);
+    if (tmpl[i].re != 0 || tmpl[i].im != 0) area += (R)1.0;
+  }
+  *area_out = area / (R)N;
+  memcpy(sorted_out, abs_out, sizeof(R) * N);
+  qsort(sorted_out, N, sizeof(R), cmp_desc);
+  free(bl);
+}
+
+/* src/support_update.c:181-207: threshold at index (int)(N * area * fraction)
+ * of the sorted template, computed in R (clamped to the array) */
+EXPORT void or_support_template_update(OrPhaser *p, const R *tmpl_abs, const R *sorted, R area, R fraction) {
+  long k = (long)(int)((R)p->N * area * fraction);
+  if (k >= p->N) k = p->N - 1;
+  if (k < 0) k = 0;
+  mask_from_threshold(p, tmpl_abs, sorted[k]);
+}
+
+static int flag_at(const int *f, int nx, int ny, int x, int y, int z) {
+  return f[((long)z * ny + y) * nx + x] & SUPPORT;
+}
+
+/* src/support_update.c:213-280: `size` dilations, then `size` erosions, each
+ * from one buffer into the other */
+EXPORT void or_support_close_update(OrPhaser *p, int size) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  int *a = (int *)malloc(sizeof(int) * p->N), *b = (int *)malloc(sizeof(int) * p->N);
+  memcpy(a, p->flags, sizeof(int) * p->N);
+  memcpy(b, p->flags, sizeof(int) * p->N);
+  for (int pass = 0; pass < 2 * size; ++pass) {
+    const int dilate = pass < size;
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y)
+        for (int z = 0; z < nz; ++z) {
+          int n[6], k = 0;
+          if (x != 0) n[k++] = flag_at(a, nx, ny, x - 1, y, z);
+          if (x != nx - 1) n[k++] = flag_at(a, nx, ny, x + 1, y, z);
+          if (y != 0) n[k++] = flag_at(a, nx, ny, x, y - 1, z);
+          if (y != ny - 1) n[k++] = flag_at(a, nx, ny, x, y + 1, z);
+          if (z != 0) n[k++] = flag_at(a, nx, ny, x, y, z - 1);
+          if (z != nz - 1) n[k++] = flag_at(a, nx, ny, x, y, z + 1);
+          int v = flag_at(a, nx, ny, x, y, z) != 0;
+          for (int q = 0; q < k; ++q) v = dilate ? (v || n[q]) : (v && n[q]);
+          const long i = ((long)z * ny + y) * nx + x;
+          b[i] = v ? (b[i] | SUPPORT) : (b[i] & ~SUPPORT);
+        }
+    int *t = a;
+    a = b;
+    b = t;
+  }
+  memcpy(p->flags, a, sizeof(int) * p->N);
+  free(a);
+  free(b);
+}
+
+/* sp_pixel_flags_translate_mask (src/image_util.c:3587-3611): the support
+ * bit moves cyclically by (dx, dy, dz); the other bits stay in place */
+static void translate_support(OrPhaser *p, int dx, int dy, int dz) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  while (dx < 0) dx += nx;
+  while (dy < 0) dy += ny;
+  while (dz < 0) dz += nz;
+  int *t = (int *)malloc(sizeof(int) * p->N);
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const long src = ((long)z * ny + y) * nx + x;
+        const long dst = ((long)((z + dz) % nz) * ny + (y + dy) % ny) * nx + (x + dx) % nx;
+        t[dst] = (p->flags[src] & SUPPORT) ? (p->flags[dst] | SUPPORT) : (p->flags[dst] & ~SUPPORT);
+      }
+  memcpy(p->flags, t, sizeof(int) * p->N);
+  free(t);
+}
+
+/* src/support_update.c:282-389 on the support bits (the reference also
+ * translates its cached copy of the model, which the next model read
+ * replaces; the iterate does not move).  The centre of mass follows
+ * sp_i3matrix_binary_center_of_mass_shifted (src/linear_alg.c:444-475):
+ * float sums in raster order, then truncation to int. */
+EXPORT void or_support_centre_image(OrPhaser *p) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  if (nz == 1) {
+    for (int ix = 0; ix < nx; ++ix)
+      if (flag_at(p->flags, nx, ny, ix, (ny - 1) / 2, 0)) {
+        translate_support(p, 0, (ny - 1) / 2, 0);
+        break;
+      }
+    for (int iy = 0; iy < ny; ++iy)
+      if (flag_at(p->flags, nx, ny, (nx - 1) / 2, iy, 0)) {
+        translate_support(p, (nx - 1) / 2, 0, 0);
+        break;
+      }
+  } else {
+    /* each break leaves the inner loop only, as in the reference */
+    for (int iy = 0; iy < ny; ++iy)
+      for (int iz = 0; iz < nz; ++iz)
+        if (flag_at(p->flags, nx, ny, (nx - 1) / 2, iy, iz)) {
+          translate_support(p, (nx - 1) / 2, 0, 0);
+          break;
+        }
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iz = 0; iz < nz; ++iz)
+        if (flag_at(p->flags, nx, ny, ix, (ny - 1) / 2, iz)) {
+          translate_support(p, 0, (ny - 1) / 2, 0);
+          break;
+        }
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy)
+        if (flag_at(p->flags, nx, ny, ix, iy, (nz - 1) / 2)) {
+          translate_support(p, 0, 0, (nz - 1) / 2);
+          break;
+        }
+  }
+  float c[3] = {0.f, 0.f, 0.f};
+  int sum = 0;
+  long i = 0;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x, ++i)
+        if (p->flags[i] & SUPPORT) {
+          c[0] += (float)((x + (nx - 1) / 2) % nx);
+          c[1] += (float)((y + (ny - 1) / 2) % ny);
+          c[2] += (float)((z + (nz - 1) / 2) % nz);
+          sum++;
+        }
+  if (sum) {
+    c[0] /= (float)sum;
+    c[1] /= (float)sum;
+    c[2] /= (float)sum;
+    c[0] -= (float)((nx - 1) / 2);
+    c[1] -= (float)((ny - 1) / 2);
+    c[2] -= (float)((nz - 1) / 2);
+  }
+  translate_support(p, (int)(0.f - c[0]), (int)(0.f - c[1]), (int)(0.f - c[2]));
+}
+
 static void support_update(OrPhaser *p, const OrSup *s) {
   if (s->type == 5) return;
+  if (s->type == 6) {
+    or_support_close_update(p, s->close_size);
+    return;
+  }
+  if (s->type == 3) {
+    or_support_centre_image(p);
+    return;
+  }
   const R radius = or_bezier(s->blur_keys, s->blur_vals, s->nblur, (R)p->iteration);
   const R v = or_bezier(s->val_keys, s->val_vals, s->nval, (R)p->iteration);
   if (s->type == 2) or_support_area_update(p, radius, v);
   else if (s->type == 1) or_support_threshold_update(p, radius, v);
+  else if (s->type == 4) or_support_template_update(p, s->tmpl_abs, s->tmpl_sorted, v, s->tmpl_area);
 }
 
-/* the reference's support scheduler (src/phasing.c:1000-1022) */
-EXPORT int or_iterate(OrPhaser *p, const OrAlg *alg, const OrSup *sup, int iterations) {
+/* the reference's support scheduler (src/phasing.c:1000-1022) over an array
+ * of nsup support algorithms run in order (src/support_update.c:489-494);
+ * the period is the first one's */
+EXPORT int or_iterate_multi(OrPhaser *p, const OrAlg *alg, const OrSup *sup, int nsup, int iterations) {
   int ret = 0;
-  if (sup && sup->type != 0) {
+  if (sup && nsup > 0 && sup->type != 0) {
     while (iterations) {
       int to_update = sup->period - 1 - p->iteration % sup->period;
       const int to_iterate = iterations < to_update ? iterations : to_update;
@@ -577,7 +749,7 @@ EXPORT int or_iterate(OrPhaser *p, const OrAlg *alg, const OrSup *sup, int itera
       if (to_update == 0 && iterations > 0) {
         if (p->nsup < 256) p->sup_log[p->nsup] = p->iteration;
         p->nsup++;
-        support_update(p, sup);
+        for (int k = 0; k < nsup; ++k) support_update(p, &sup[k]);
         p->iteration++;
         iterations -= 1;
       }
@@ -586,6 +758,10 @@ EXPORT int or_iterate(OrPhaser *p, const OrAlg *alg, const OrSup *sup, int itera
     ret = run_alg(p, alg, iterations);
   }
   return ret;
+}
+
+EXPORT int or_iterate(OrPhaser *p, const OrAlg *alg, const OrSup *sup, int iterations) {
+  return or_iterate_multi(p, alg, sup, sup ? 1 : 0, iterations);
 }
 
 /* ------------------------------------------------------------------------ */
