@@ -15,8 +15,8 @@ pass, 37 N for the Fourier-space pass), the whole-iteration roofline, the
 1024^2 RAAR + shrinkwrap reconstruction (300 iterations, area support every
 20), the batched workload (128 patterns x 8 starts), the served workload
 (Poisson 200 req/s), the 256^3 DM + ER reconstruction, the Reconstructor
-front-end's reconstruct_loop, and a CPU baseline: the oracle's restatement of
-the reference loop on one host core.
+front-end's reconstruct_loop, the PRTF over many reconstructions, and a CPU
+baseline: the oracle's restatement of the reference loop on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                     [--no-batched] [--no-serving] [--no-volume]
@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batched", action="store_true", help="skip the many-reconstruction workload")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the many-reconstruction workloads (batched, Reconstructor, PRTF)")
     ap.add_argument("--no-serving", action="store_true", help="skip the served workload")
     ap.add_argument("--no-volume", action="store_true", help="skip the 256^3 reconstruction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -336,6 +337,51 @@ def reconstructor_bench(repeats=32, single_sample=4, seed=600):
             "speedup": t_single * repeats / t_loop, "final_errors_equal_in_sample": same}
 
 
+def prtf_bench(shape=(256, 256), counts=(32, 128), seed=700):
+    """PRTF over many reconstructions (the reference's prtf(), translate and
+    enantio on): noisy copies of one complex object with random shifts,
+    reflections and constant phases.  The GPU path against the oracle's
+    numpy restatement of the same algorithm on one host core (its
+    alignment decisions must agree: the aligned masks are compared)."""
+    import numpy as np
+    from libspimage_amd import prtf as gp
+    from oracle import prtf_oracle as P
+    rng = np.random.default_rng(seed)
+    obj = np.zeros(shape, complex)
+    sl = tuple(slice(n // 4, n // 4 + n // 3) for n in shape)
+    blk = obj[sl].shape
+    obj[sl] = (rng.random(blk) + 0.3) * np.exp(0.5j * rng.random(blk))
+    sup = (np.abs(obj) > 0).astype(np.int32)
+    nmax = max(counts)
+    imgs = np.empty((nmax,) + shape, np.complex64)
+    masks = np.empty((nmax,) + shape, np.int32)
+    for k in range(nmax):
+        b = np.flip(obj) if k % 3 == 2 else obj
+        b = b * np.exp(1j * rng.uniform(-3, 3)) + 0.01 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+        s = tuple(int(v) for v in rng.integers(-8, 9, len(shape)))
+        imgs[k] = np.roll(b, s, axis=(0, 1))
+        masks[k] = np.roll(sup, s, axis=(0, 1))
+    gp.prtf(imgs[:4], masks[:4])  # warm-up (tuning, code objects)
+    res = {"workload": f"PRTF of noisy shifted / reflected / phase-shifted copies, {shape[0]}x{shape[1]}, "
+                       "translate + enantio", "runs": []}
+    for n in counts:
+        t0 = time.perf_counter()
+        g = gp.prtf(imgs[:n], masks[:n], full_out=True)
+        t = time.perf_counter() - t0
+        res["runs"].append({"reconstructions": n, "seconds": t, "reconstructions_per_s": n / t})
+    n0 = counts[0]
+    t0 = time.perf_counter()
+    o = P.prtf(imgs[:n0], masks[:n0], full_out=True)
+    t_cpu = time.perf_counter() - t0
+    g = gp.prtf(imgs[:n0], masks[:n0], full_out=True)
+    res["cpu_oracle"] = {"reconstructions": n0, "seconds": t_cpu, "cores": 1,
+                         "kind": "port (numpy float64 restatement of prtf())"}
+    res["speedup_vs_cpu"] = t_cpu / res["runs"][0]["seconds"]
+    res["aligned_masks_equal"] = bool(np.array_equal(g["masks"], o["masks"]))
+    res["max_prtf_difference"] = float(np.abs(g["prtf"] - o["prtf"]).max())
+    return res
+
+
 def _serving_pattern(args):
     from libspimage_amd import workloads
     shape, seed = args
@@ -469,6 +515,7 @@ def main():
     serving = None if args.no_serving else serving_bench()
     volume = None if args.no_volume else volume_bench()
     reconstructor = None if args.no_batched else reconstructor_bench()
+    prtf = None if args.no_batched else prtf_bench()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
@@ -492,6 +539,7 @@ def main():
         "serving": serving,
         "volume_256": volume,
         "reconstructor": reconstructor,
+        "prtf": prtf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -57,6 +57,9 @@ struct Shape {
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
+// unnormalised transform of every reconstruction of `sh` (dir < 0 forward,
+// e^{-2 pi i k.n/N}); in == out is allowed
+void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s);
 
 class Engine {
  public:

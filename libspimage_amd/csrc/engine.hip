@@ -714,7 +714,7 @@ static RowArgs row_args(const Shape& sh, const float2* in, float2* out) {
 }
 
 // FFT over all non-trivial axes: x by a row pass, then y and z by column passes
-static void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
+void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
   const float2* src = in;
   if (sh.ex) {
     sh.ex->row(dir < 0 ? kRowFwd : kRowInv, row_args(sh, in, out), s);

@@ -290,6 +290,9 @@ _bind("spx_batch_query", C.c_int, PBatch)
 _bind("spx_batch_estimate_bytes", C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_device_memory", C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
 
+# PRTF (include/sp_prtf.h)
+_bind("spx_prtf", C.c_int, PF, PI, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, PF, PF, PF, PI, PF)
+
 
 def device_memory() -> tuple[int, int]:
     """(free, total) bytes of device memory."""
