@@ -28,6 +28,11 @@ spimage_EXPORT int spx_device_available(void);
  * (hipMemGetInfo). */
 spimage_EXPORT int spx_device_memory(unsigned long long *free_bytes, unsigned long long *total_bytes);
 spimage_EXPORT const char *spx_last_error(void);
+/* Page-locked host memory (hipHostMalloc) for staging large downloads,
+ * which then move at the full link rate; NULL when it cannot be allocated.
+ * No counterpart in the reference. */
+spimage_EXPORT void *spx_host_alloc(unsigned long long bytes);
+spimage_EXPORT void spx_host_free(void *p);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
 /* Elements per thread (4, 8 or 16) of the transform variant the tuner chose

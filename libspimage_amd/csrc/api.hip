@@ -1167,6 +1167,24 @@ int spx_device_memory(unsigned long long* free_b, unsigned long long* total_b) {
   return 0;
 }
 
+void* spx_host_alloc(unsigned long long bytes) {
+  if (!spx::device_available()) {
+    spx::set_error("no HIP device available");
+    return nullptr;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, (size_t)(bytes ? bytes : 1), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    spx::set_error("hipHostMalloc(%llu): %s", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  return p;
+}
+
+void spx_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
 int spx_fft_variant(int L, int tiled, int* row_e, int* col_e) {

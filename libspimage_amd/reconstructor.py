@@ -43,6 +43,24 @@ _PHASING_KWARGS = {"raar": ["beta_init", "beta_final"], "hio": ["beta_init", "be
                    "diffmap": ["beta_init", "beta_final", "gamma1", "gamma2"]}
 
 
+
+def _shift_into(dst, src, axes):
+    """dst[...] = np.fft.fftshift(src, axes) in one pass: the shift (a roll by
+    n // 2 on each axis) as block copies, converting to dst's type on the way."""
+    import itertools
+    blocks = []
+    for ax in axes:
+        n = src.shape[ax]
+        h = n // 2
+        # (destination slice, source slice) of the two halves of the roll
+        blocks.append(((slice(h, n), slice(0, n - h)), (slice(0, h), slice(n - h, n))))
+    for combo in itertools.product(*blocks):
+        d = [slice(None)] * src.ndim
+        s_ = [slice(None)] * src.ndim
+        for ax, (ds, ss) in zip(axes, combo):
+            d[ax], s_[ax] = ds, ss
+        dst[tuple(d)] = src[tuple(s_)]
+
 class Reconstructor:
     def __init__(self, use_gpu=True):
         if not use_gpu:
@@ -357,11 +375,11 @@ class Reconstructor:
                 if cs is not None:
                     b.set_support_schedule(supports[cs][1])
                 if oi:
-                    img = b.model_before_projection() if it > 0 else b.model()
+                    img = b.model_before_projection(staged=True) if it > 0 else b.model()
+                    _shift_into(real_space[:, k_img], img, axes)
                     msk = b.support() != 0 if it > 0 else np.broadcast_to(S != 0, (n,) + shape)
-                    real_space[:, k_img] = np.fft.fftshift(img, axes=axes)
-                    support[:, k_img] = np.fft.fftshift(msk, axes=axes)
-                    fourier_space[:, k_img] = np.fft.fftshift(b.fmodel(), axes=axes)
+                    _shift_into(support[:, k_img], msk, axes)
+                    _shift_into(fourier_space[:, k_img], b.fmodel(staged=True), axes)
                     k_img += 1
                 if os_:
                     ef, _ = b.efourier()

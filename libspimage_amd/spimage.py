@@ -289,6 +289,8 @@ _bind("spx_batch_get_fmodel", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_query", C.c_int, PBatch)
 _bind("spx_batch_estimate_bytes", C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_device_memory", C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
+_bind("spx_host_alloc", C.c_void_p, C.c_ulonglong)
+_bind("spx_host_free", None, C.c_void_p)
 
 # PRTF (include/sp_prtf.h)
 _bind("spx_prtf", C.c_int, PF, PI, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, PF, PF, PF, PI, PF)
@@ -324,11 +326,29 @@ class Batch:
         if not self.h:
             raise RuntimeError(f"spx_batch_create failed: {last_error()}")
         self._keep = []
+        self._stage = None  # page-locked staging for the downloads of all reconstructions
 
     def close(self):
         if self.h:
             spx_batch_free(self.h)
             self.h = None
+        if self._stage is not None:
+            spx_host_free(self._stage[0])
+            self._stage = None
+
+    def _staging(self, dtype):
+        """A view of the page-locked staging buffer as nrec arrays of `dtype`,
+        or None when page-locked memory is not available."""
+        nbytes = self.nrec * int(np.prod(self.shape)) * np.dtype(dtype).itemsize
+        if self._stage is None or self._stage[1] < nbytes:
+            if self._stage is not None:
+                spx_host_free(self._stage[0])
+            p = spx_host_alloc(nbytes)
+            self._stage = (p, nbytes) if p else None
+            if p is None:
+                return None
+        buf = (C.c_char * nbytes).from_address(self._stage[0])
+        return np.frombuffer(buf, dtype=dtype).reshape((self.nrec,) + self.shape)
 
     def __del__(self):
         self.close()
@@ -398,17 +418,22 @@ class Batch:
         self._check(spx_batch_get_model(self.h, rec, out.ctypes.data_as(PF)), "get_model")
         return out if rec < 0 else out[0]
 
-    def _complex(self, fn, rec, what):
+    def _complex(self, fn, rec, what, staged=False):
         n = self.nrec if rec < 0 else 1
-        out = np.empty((n,) + self.shape, np.complex64)
+        out = self._staging(np.complex64) if (staged and rec < 0) else None
+        if out is None:
+            out = np.empty((n,) + self.shape, np.complex64)
         self._check(fn(self.h, rec, out.ctypes.data_as(PF)), what)
         return out if rec < 0 else out[0]
 
-    def model_before_projection(self, rec=-1):
-        return self._complex(spx_batch_get_model_before_projection, rec, "get_model_before_projection")
+    # staged=True: the result of all reconstructions lands in page-locked
+    # staging memory and is returned as a view of it, valid until the next
+    # staged call (copy it out before then)
+    def model_before_projection(self, rec=-1, staged=False):
+        return self._complex(spx_batch_get_model_before_projection, rec, "get_model_before_projection", staged)
 
-    def fmodel(self, rec=-1):
-        return self._complex(spx_batch_get_fmodel, rec, "get_fmodel")
+    def fmodel(self, rec=-1, staged=False):
+        return self._complex(spx_batch_get_fmodel, rec, "get_fmodel", staged)
 
     def support(self, rec=-1):
         n = self.nrec if rec < 0 else 1
