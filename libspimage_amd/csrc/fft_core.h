@@ -6,8 +6,8 @@
 // (X_k = sum_n x_n e^{-2 pi i k n / L}), inverse sign +1, row-major with x
 // fastest.
 //
-// A length-L line is split into radix stages (2, 3, 4, 5, 8, 16) chosen at
-// compile time.  T threads cooperate on a line.  Stage 0 reads its inputs from
+// A length-L line is split into radix stages (2, 3, 4, 5, 6, 8, 9, 10, 12,
+// 15, 16) chosen at compile time.  T threads cooperate on a line.  Stage 0 reads its inputs from
 // registers (loaded straight from global memory: butterfly b takes the
 // elements b + k L/R0, so consecutive threads read consecutive addresses),
 // the middle stages go through LDS (two buffers, one barrier per stage) and
@@ -65,9 +65,72 @@ constexpr void push_pow2(FftPlan& p, int m, int kmax) {
   }
 }
 
+// Radices a stage can have: powers of two up to 16, 3, 5, and the products
+// 6, 9, 10, 12, 15 done in registers by one Cooley-Tukey step (dft_ct).
+constexpr int kMixRadices[11] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+
+// The fewest radices (at most `depth`) from kMixRadices whose product is m,
+// largest first, into out; returns their number, or 99 if there are none.
+constexpr int split_radices(int m, int depth, int* out) {
+  if (m == 1) return 0;
+  int cap = 1;
+  for (int d = 0; d < depth && cap < m; ++d) cap *= 16;
+  if (depth == 0 || cap < m) return 99;
+  int best = 99;
+  int bestv[8] = {};
+  for (int i = 0; i < 11; ++i) {
+    const int r = kMixRadices[i];
+    if (m % r != 0) continue;
+    int sub[8] = {};
+    const int c = split_radices(m / r, depth - 1, sub);
+    if (c + 1 < best) {
+      best = c + 1;
+      bestv[0] = r;
+      for (int k = 0; k < c; ++k) bestv[k + 1] = sub[k];
+    }
+  }
+  if (best < 99)
+    for (int k = 0; k < best; ++k) out[k] = bestv[k];
+  return best;
+}
+
+// Palindromic plan e, (fewest middle radices), e with e^2 | L: among the
+// edge radices within a factor 2 of E the one with the fewest stages, ties
+// going to the one closest to E; if none is that close, the closest.  False
+// when no radix e >= 2 has e^2 | L.
+constexpr bool mixed_plan(int L, int E, FftPlan& p) {
+  int best_e = 0, best_c = 99, best_key = 1 << 30;
+  int best_mid[8] = {};
+  for (int i = 0; i < 11; ++i) {
+    const int e = kMixRadices[i];
+    if (L % (e * e) != 0) continue;
+    int mid[8] = {};
+    int c = 99;
+    for (int d = 0; d <= 6 && c == 99; ++d) c = split_radices(L / (e * e), d, mid);
+    if (c == 99) continue;
+    const int ratio = e >= E ? (e * 1024) / E : (E * 1024) / e;  // 1024 = equal
+    const int key = ratio <= 2048 ? c * 4096 + ratio : (1 << 20) + ratio;
+    if (key < best_key) {
+      best_e = e;
+      best_c = c;
+      best_key = key;
+      for (int k = 0; k < c; ++k) best_mid[k] = mid[k];
+    }
+  }
+  if (!best_e) return false;
+  p.n = 0;
+  p.r[p.n++] = best_e;
+  for (int k = 0; k < best_c; ++k) p.r[p.n++] = best_mid[k];
+  p.r[p.n++] = best_e;
+  return true;
+}
+
 // Plan of a length-L line whose threads hold E elements at the edges
 // (E = 4, 8 or 16; also the largest power-of-two radix used).  The edge
-// radix 2^ke appears first and last so the transform is palindromic.
+// radix 2^ke appears first and last so the transform is palindromic.  When
+// L's power-of-two part is too small for edges of E (a line of 2^a 3^b 5^c
+// with 2^a < E^2), the plan of mixed radices with edges closest to E
+// replaces it if it has fewer stages: 1000 = 2 2 5 5 5 2 becomes 10 10 10.
 constexpr FftPlan make_plan(int L, int E) {
   FftPlan p{};
   p.L = L;
@@ -96,6 +159,13 @@ constexpr FftPlan make_plan(int L, int E) {
       for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
       for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
     }
+    if (n3 + n5 > 0 && ke < kE) {
+      FftPlan q = p;
+      if (mixed_plan(L, E, q) && q.n < p.n) {
+        p.n = q.n;
+        for (int s = 0; s < q.n; ++s) p.r[s] = q.r[s];
+      }
+    }
   }
   int ns = 1, off = 0, mr = 1;
   for (int s = 0; s < p.n; ++s) {
@@ -116,7 +186,7 @@ constexpr FftPlan make_plan(int L, int E) {
 template <int L, int E>
 struct PlanOf {
   static constexpr FftPlan P = make_plan(L, E);
-  static_assert(P.ok, "transform length does not factor into radices 2,3,4,5,8,16");
+  static_assert(P.ok, "transform length does not factor into the supported radices");
   static constexpr int T = P.T;
   static constexpr int E0 = P.r[0];
   static constexpr int EL = P.r[P.n - 1];
@@ -261,12 +331,100 @@ __device__ __forceinline__ void dft5(float2* x) {
 }
 
 template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* x);
+
+// (cos, sin) of 2 pi m / R, 0 < m < R, for the composite radices
+template <int R>
+__device__ __forceinline__ float2 unit_root(int m) {
+  if constexpr (R == 6) {
+    constexpr float c[6] = {1.f, 0.5f, -0.5f, -1.f, -0.5f, 0.5f};
+    constexpr float s[6] = {0.f, 0.86602540378443864676f, 0.86602540378443864676f, 0.f, -0.86602540378443864676f,
+                            -0.86602540378443864676f};
+    return make_float2(c[m], s[m]);
+  } else if constexpr (R == 9) {
+    constexpr float c[9] = {1.f, 0.76604444311897803520f, 0.17364817766693034885f, -0.5f, -0.93969262078590838405f,
+                            -0.93969262078590838405f, -0.5f, 0.17364817766693034885f, 0.76604444311897803520f};
+    constexpr float s[9] = {0.f, 0.64278760968653932632f, 0.98480775301220805936f, 0.86602540378443864676f,
+                            0.34202014332566873304f, -0.34202014332566873304f, -0.86602540378443864676f,
+                            -0.98480775301220805936f, -0.64278760968653932632f};
+    return make_float2(c[m], s[m]);
+  } else if constexpr (R == 10) {
+    constexpr float c[10] = {1.f, 0.80901699437494742410f, 0.30901699437494742410f, -0.30901699437494742410f,
+                             -0.80901699437494742410f, -1.f, -0.80901699437494742410f, -0.30901699437494742410f,
+                             0.30901699437494742410f, 0.80901699437494742410f};
+    constexpr float s[10] = {0.f, 0.58778525229247312917f, 0.95105651629515357212f, 0.95105651629515357212f,
+                             0.58778525229247312917f, 0.f, -0.58778525229247312917f, -0.95105651629515357212f,
+                             -0.95105651629515357212f, -0.58778525229247312917f};
+    return make_float2(c[m], s[m]);
+  } else if constexpr (R == 12) {
+    constexpr float c[12] = {1.f, 0.86602540378443864676f, 0.5f, 0.f, -0.5f, -0.86602540378443864676f, -1.f,
+                             -0.86602540378443864676f, -0.5f, 0.f, 0.5f, 0.86602540378443864676f};
+    constexpr float s[12] = {0.f, 0.5f, 0.86602540378443864676f, 1.f, 0.86602540378443864676f, 0.5f, 0.f, -0.5f,
+                             -0.86602540378443864676f, -1.f, -0.86602540378443864676f, -0.5f};
+    return make_float2(c[m], s[m]);
+  } else {
+    static_assert(R == 15, "no table for this radix");
+    constexpr float c[15] = {1.f, 0.91354545764260089550f, 0.66913060635885821383f, 0.30901699437494742410f,
+                             -0.10452846326765347140f, -0.5f, -0.80901699437494742410f, -0.97814760073380563793f,
+                             -0.97814760073380563793f, -0.80901699437494742410f, -0.5f, -0.10452846326765347140f,
+                             0.30901699437494742410f, 0.66913060635885821383f, 0.91354545764260089550f};
+    constexpr float s[15] = {0.f, 0.40673664307580020775f, 0.74314482547739423501f, 0.95105651629515357212f,
+                             0.99452189536827333692f, 0.86602540378443864676f, 0.58778525229247312917f,
+                             0.20791169081775933710f, -0.20791169081775933710f, -0.58778525229247312917f,
+                             -0.86602540378443864676f, -0.99452189536827333692f, -0.95105651629515357212f,
+                             -0.74314482547739423501f, -0.40673664307580020775f};
+    return make_float2(c[m], s[m]);
+  }
+}
+
+// DFT of R = R1 R2 points in registers by one Cooley-Tukey step: R2-point
+// DFTs of the R1 subsequences x[n1 + R1 n2], the twiddles
+// exp(DIR 2 pi i n1 k2 / R), then R1-point DFTs; X[R2 k1 + k2] comes out in
+// natural order.
+template <int R1, int R2, int DIR>
+__device__ __forceinline__ void dft_ct(float2* x) {
+  constexpr int R = R1 * R2;
+  float2 y[R];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    float2 c[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) c[n2] = x[n1 + R1 * n2];
+    dft<R2, DIR>(c);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const int m = n1 * k2;
+      if (m == 0) {
+        y[n1 * R2 + k2] = c[k2];
+      } else {
+        const float2 w = unit_root<R>(m);
+        y[n1 * R2 + k2] = make_float2(c[k2].x * w.x - DIR * c[k2].y * w.y, c[k2].y * w.x + DIR * c[k2].x * w.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    float2 r[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) r[n1] = y[n1 * R2 + k2];
+    dft<R1, DIR>(r);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) x[R2 * k1 + k2] = r[k1];
+  }
+}
+
+template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* x) {
   if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
   else if constexpr (R == 3) dft3<DIR>(x);
   else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
   else if constexpr (R == 5) dft5<DIR>(x);
+  else if constexpr (R == 6) dft_ct<2, 3, DIR>(x);
   else if constexpr (R == 8) dft8<DIR>(x);
+  else if constexpr (R == 9) dft_ct<3, 3, DIR>(x);
+  else if constexpr (R == 10) dft_ct<2, 5, DIR>(x);
+  else if constexpr (R == 12) dft_ct<4, 3, DIR>(x);
+  else if constexpr (R == 15) dft_ct<3, 5, DIR>(x);
   else if constexpr (R == 16) dft16<DIR>(x);
   else static_assert(R == 2, "unsupported radix");
 }

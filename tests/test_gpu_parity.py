@@ -23,7 +23,10 @@ ONE_ITER_TOL = 1e-5
 # transforms
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("shape", [(8, 8), (16, 16), (48, 64), (96, 80), (384, 384), (1024, 1024),
-                                   (125, 60), (16, 16, 16), (64, 48, 80), (20, 36, 30)])
+                                   (125, 60), (16, 16, 16), (64, 48, 80), (20, 36, 30),
+                                   # mixed-radix plans (6, 9, 10, 12, 15 stages): 10.10.10, 9.12.9,
+                                   # 10.9.10, 15.3.15, 9.9.9, 12.5.12, 15.15, 5.15.2.5, 10.10
+                                   (1000, 972), (900, 675), (729, 720), (225, 750), (100, 90, 96)])
 def test_fft_matches_oracle(sp, o64, shape):
     rng = np.random.default_rng(sum(shape))
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
@@ -78,7 +81,7 @@ def oracle_run(o, alg, amp, meas, sup, g0, n=1, beta=0.9, constraints=0, gammas=
 
 @pytest.mark.parametrize("alg", ["er", "hio", "raar", "dm"])
 @pytest.mark.parametrize("c", CONSTRAINTS)
-@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (16, 16, 16)])
+@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (16, 16, 16), (90, 100)])
 def test_one_iteration(sp, o32, alg, c, shape):
     # 64 x 64 runs in the row-major layout, 128 x 128 in the 4-column tiled
     # layout of the Fourier-space arrays, 16^3 is 3-D
