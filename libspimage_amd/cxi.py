@@ -178,6 +178,12 @@ class H5File:
     def __exit__(self, *exc):
         self.close()
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def __contains__(self, path) -> bool:
         parts = [p for p in path.split("/") if p]
         cur = ""
