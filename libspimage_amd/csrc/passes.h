@@ -346,6 +346,10 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
 // lines whose lanes interleave as 16 positions x 4 lines, so that one
 // wavefront load covers 16 consecutive x of 4 rows: four whole 128-byte
 // lines of a row-major array and four whole 4 x 4 tiles of the tiled one.
+// LDS for the two line buffers of a work-group: gfx950 gives a work-group
+// up to 160 KiB; 4 KiB stay for the kernels' reduction scratch
+constexpr int kLdsBytes = 156 * 1024;
+
 template <int L, int E, bool T4>
 struct RowGeo {
   static constexpr int T = PlanOf<L, E>::T;
@@ -357,11 +361,11 @@ struct RowGeo {
   }
 };
 // the tiled row kernels need T % 16 == 0, 4 T <= 1024 threads and two LDS
-// buffers of 4 lines within 96 KiB
+// buffers of 4 lines within the 160 KiB of LDS a gfx950 work-group may use
 template <int L, int E>
 struct T4Ok {
   static constexpr int T = PlanOf<L, E>::T;
-  static constexpr bool value = (T % 16 == 0) && (4 * T <= 1024) && (2 * LdsView<L, 4, false>::SIZE * 8 <= 98304);
+  static constexpr bool value = (T % 16 == 0) && (4 * T <= 1024) && (2 * LdsView<L, 4, false>::SIZE * 8 <= kLdsBytes);
 };
 
 template <int L, int E, bool DM, bool T4>
@@ -534,10 +538,10 @@ template <int L, int E>
 struct ColShape {
   static constexpr int T = PlanOf<L, E>::T;
   // C >= 4 adjacent lines (32-byte segments) and about 256 threads, within
-  // 1024 threads and about 96 KiB of LDS
+  // 1024 threads and the work-group's LDS
   static constexpr int LS = L + (L >> 4);
   static constexpr int Cthr = (1024 / T) >= 1 ? (1024 / T) : 1;
-  static constexpr int Cmem = (98304 / (2 * LS * 8)) >= 1 ? (98304 / (2 * LS * 8)) : 1;
+  static constexpr int Cmem = (kLdsBytes / (2 * LS * 8)) >= 1 ? (kLdsBytes / (2 * LS * 8)) : 1;
   static constexpr int Cw = (256 / T) > 4 ? (256 / T) : 4;
   static constexpr int C1 = Cw < 64 ? Cw : 64;
   static constexpr int C2 = C1 < Cthr ? C1 : Cthr;

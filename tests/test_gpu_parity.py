@@ -26,7 +26,9 @@ ONE_ITER_TOL = 1e-5
                                    (125, 60), (16, 16, 16), (64, 48, 80), (20, 36, 30),
                                    # mixed-radix plans (6, 9, 10, 12, 15 stages): 10.10.10, 9.12.9,
                                    # 10.9.10, 15.3.15, 9.9.9, 12.5.12, 15.15, 5.15.2.5, 10.10
-                                   (1000, 972), (900, 675), (729, 720), (225, 750), (100, 90, 96)])
+                                   (1000, 972), (900, 675), (729, 720), (225, 750), (100, 90, 96),
+                                   # long lines: the column passes hold 4 or 2 lines of 2048 / 4096
+                                   (2048, 1536), (4096, 512)])
 def test_fft_matches_oracle(sp, o64, shape):
     rng = np.random.default_rng(sum(shape))
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
@@ -176,6 +178,15 @@ def test_tiled_layout_equals_row_major_layout(sp):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+def test_hio_iteration_2048(sp, o32):
+    """2048 x 2048 (tiled layout with 4 lines of 2048 per work-group) against the oracle."""
+    obj, amp, meas, sup, g0 = random_case((2048, 2048), seed=5, fill=0.1)
+    with GpuPhaser(sp, "hio", amp, meas, sup, g0, beta=0.9) as gph:
+        assert gph.iterate(1) == 0
+        oph = oracle_run(o32, "hio", amp, meas, sup, g0, beta=0.9)
+        assert rel(gph.get("model"), oph.g1) < ONE_ITER_TOL
 
 
 def test_full_size_hio_iteration_1024(sp, o32):
