@@ -438,6 +438,21 @@ __device__ __forceinline__ void dft(float2* x) {
 // One padding slot after every 16 positions: the stage-0 writes (stride =
 // radix, up to 16) and the early middle-stage writes then spread over the
 // banks instead of piling onto one.
+// LDS for the line buffers of a work-group: gfx950 gives a work-group up to
+// 160 KiB; 4 KiB stay for the kernels' reduction scratch
+constexpr int kLdsBytes = 156 * 1024;
+
+// Lines too long for two buffers of 4 lines each run their stages in one
+// buffer: a stage loads all its inputs, synchronises, then stores (two
+// barriers per stage instead of one), which halves the LDS and keeps 4 lines
+// per work-group for the column passes and the tiled layout.
+template <int L>
+struct SingleBuf {
+  static constexpr int LS = L + (L >> 4);
+  static constexpr bool value = 2 * 4 * (LS + 1) * 8 > kLdsBytes;
+  static constexpr int buffers = value ? 1 : 2;
+};
+
 template <int L, int G, bool INTERLEAVED>
 struct LdsView {
   static constexpr int slot(int pos) { return pos + (pos >> 4); }
@@ -504,10 +519,49 @@ __device__ __forceinline__ void mid_stage(const View& in, const View& out, int t
   }
 }
 
+// Middle stage S in one buffer: every input of the thread's butterflies is
+// read, the work-group synchronises, then the outputs are written over them.
+template <int L, int E, int S, int DIR, class View>
+__device__ __forceinline__ void mid_stage_inplace(const View& buf, int t, const TwRegs<L, E>& tr) {
+  constexpr FftPlan P = PlanOf<L, E>::P;
+  constexpr int R = P.r[S], NS = P.ns[S], T = P.T;
+  constexpr int NB = L / R;
+  constexpr int BPT = PlanOf<L, E>::bpt(S), O = PlanOf<L, E>::twreg(S);
+  float2 x[BPT][R];
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = t + i * T;
+    if (BPT * T == NB || b < NB) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[i][k] = buf.at(b + k * NB);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = t + i * T;
+    if (BPT * T == NB || b < NB) {
+      const int j = b % NS;
+#pragma unroll
+      for (int k = 1; k < R; ++k) x[i][k] = twmul<DIR>(x[i][k], tr.w[O + i * (R - 1) + (k - 1)]);
+      dft<R, DIR>(x[i]);
+      const int base = (b - j) * R + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf.at(base + r * NS) = x[i][r];
+    }
+  }
+}
+
 template <int L, int E, int S, int DIR, class View>
 __device__ __forceinline__ void mid_stages(const View& a, const View& b, int t, const TwRegs<L, E>& tr) {
   constexpr FftPlan P = PlanOf<L, E>::P;
-  if constexpr (S < P.n - 1) {
+  if constexpr (SingleBuf<L>::value) {
+    if constexpr (S < P.n - 1) {
+      mid_stage_inplace<L, E, S, DIR>(a, t, tr);
+      __syncthreads();
+      mid_stages<L, E, S + 1, DIR>(a, b, t, tr);
+    }
+  } else if constexpr (S < P.n - 1) {
     // stage S reads the buffer stage S-1 wrote: a for odd S, b for even S
     if constexpr (S % 2 == 1) mid_stage<L, E, S, DIR>(a, b, t, tr);
     else mid_stage<L, E, S, DIR>(b, a, t, tr);
@@ -543,7 +597,7 @@ __device__ __forceinline__ void fft_line(RegLine<L, E>& v, const View& a, const 
     // last stage reads the buffer written by stage n-2
     constexpr int SL = P.n - 1;
     constexpr int EL = P.r[SL], NS = P.ns[SL], BL = PlanOf<L, E>::BL;
-    const View& src = (SL % 2 == 1) ? a : b;
+    const View& src = (SingleBuf<L>::value || SL % 2 == 1) ? a : b;
 #pragma unroll
     for (int i = 0; i < BL; ++i) {
       const int bb = t + i * T;  // bb < NS: j = bb

@@ -263,7 +263,7 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
   constexpr int NR = PL::NREG;
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
-  __shared__ float2 shB[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
   __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
@@ -346,10 +346,6 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
 // lines whose lanes interleave as 16 positions x 4 lines, so that one
 // wavefront load covers 16 consecutive x of 4 rows: four whole 128-byte
 // lines of a row-major array and four whole 4 x 4 tiles of the tiled one.
-// LDS for the two line buffers of a work-group: gfx950 gives a work-group
-// up to 160 KiB; 4 KiB stay for the kernels' reduction scratch
-constexpr int kLdsBytes = 156 * 1024;
-
 template <int L, int E, bool T4>
 struct RowGeo {
   static constexpr int T = PlanOf<L, E>::T;
@@ -365,7 +361,7 @@ struct RowGeo {
 template <int L, int E>
 struct T4Ok {
   static constexpr int T = PlanOf<L, E>::T;
-  static constexpr bool value = (T % 16 == 0) && (4 * T <= 1024) && (2 * LdsView<L, 4, false>::SIZE * 8 <= kLdsBytes);
+  static constexpr bool value = (T % 16 == 0) && (4 * T <= 1024) && (SingleBuf<L>::buffers * LdsView<L, 4, false>::SIZE * 8 <= kLdsBytes);
 };
 
 template <int L, int E, bool DM, bool T4>
@@ -376,7 +372,7 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
   constexpr int NR = PL::NREG;
   using View = LdsView<L, M, false>;
   __shared__ float2 shA[View::SIZE];
-  __shared__ float2 shB[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
   const int tid = threadIdx.x;
   const int g = RG::line_of(tid), t = RG::pos_of(tid);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -508,7 +504,7 @@ __global__ __launch_bounds__((RowGeo<L, E, true>::NT)) void k_row_fwd_t4(RowArgs
   constexpr int NR = PlanOf<L, E>::NREG;
   using View = LdsView<L, 4, false>;
   __shared__ float2 shA[View::SIZE];
-  __shared__ float2 shB[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
   const int tid = threadIdx.x;
   const int g = RG::line_of(tid), t = RG::pos_of(tid);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -541,7 +537,8 @@ struct ColShape {
   // 1024 threads and the work-group's LDS
   static constexpr int LS = L + (L >> 4);
   static constexpr int Cthr = (1024 / T) >= 1 ? (1024 / T) : 1;
-  static constexpr int Cmem = (kLdsBytes / (2 * LS * 8)) >= 1 ? (kLdsBytes / (2 * LS * 8)) : 1;
+  static constexpr int NBUF = SingleBuf<L>::buffers;
+  static constexpr int Cmem = (kLdsBytes / (NBUF * LS * 8)) >= 1 ? (kLdsBytes / (NBUF * LS * 8)) : 1;
   static constexpr int Cw = (256 / T) > 4 ? (256 / T) : 4;
   static constexpr int C1 = Cw < 64 ? Cw : 64;
   static constexpr int C2 = C1 < Cthr ? C1 : Cthr;
@@ -556,7 +553,7 @@ __global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
   constexpr int NR = PL::NREG;
   using View = LdsView<L, C, true>;
   __shared__ float2 shA[View::SIZE];
-  __shared__ float2 shB[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
   __shared__ double red[(MODE == kColEFourier) ? 4 * 32 : 1];
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;

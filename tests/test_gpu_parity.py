@@ -27,8 +27,8 @@ ONE_ITER_TOL = 1e-5
                                    # mixed-radix plans (6, 9, 10, 12, 15 stages): 10.10.10, 9.12.9,
                                    # 10.9.10, 15.3.15, 9.9.9, 12.5.12, 15.15, 5.15.2.5, 10.10
                                    (1000, 972), (900, 675), (729, 720), (225, 750), (100, 90, 96),
-                                   # long lines: the column passes hold 4 or 2 lines of 2048 / 4096
-                                   (2048, 1536), (4096, 512)])
+                                   # long lines: two LDS buffers up to 2304, one (in-place stages) from 2400
+                                   (2048, 1536), (4096, 512), (2400, 3072), (2, 4096, 4000)])
 def test_fft_matches_oracle(sp, o64, shape):
     rng = np.random.default_rng(sum(shape))
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
