@@ -411,11 +411,14 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
                               beta=0.9 if alg == "hio" else 0.87, arrival=t if realtime else 0.0)
                 for k, (t, s, alg, its, sd) in enumerate(trace)]
 
-    # warm-up: every shape once through both paths (per-length tuning, twiddles)
-    warm = [serve.Request(-1 - i, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
-            for i, s in enumerate(shapes)]
+    # warm-up: every shape through both paths (per-length tuning, twiddles).
+    # The tuner keys its layout choice on one reconstruction or several per
+    # launch, so the server sees each shape twice (a batch of two) and the
+    # plain loop once.
+    warm = [serve.Request(-1 - 2 * i - k, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
+            for i, s in enumerate(shapes) for k in range(2)]
     plain = serve.PlainRunner()
-    for r in warm:
+    for r in warm[::2]:
         plain.run_one(r)
     srv = serve.Server()
     srv.run(warm, realtime=False)

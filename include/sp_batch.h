@@ -67,6 +67,13 @@ spimage_EXPORT int spx_batch_iteration(const SpxBatch *b);
 spimage_EXPORT int spx_batch_efourier(SpxBatch *b, float *efourier, float *fcfo);
 spimage_EXPORT int spx_batch_ereal(SpxBatch *b, float *ereal);
 spimage_EXPORT int spx_batch_support_fraction(SpxBatch *b, float *fraction);
+/* spx_batch_efourier in two halves, for a host loop that feeds several
+ * batches: _enqueue queues the error sums behind the batch's queued work and
+ * returns at once; _fetch returns them (waiting for the batch if it is still
+ * running; without waiting once spx_batch_query reports 1).  One enqueue per
+ * fetch; -1 from a fetch without an enqueue. */
+spimage_EXPORT int spx_batch_efourier_enqueue(SpxBatch *b);
+spimage_EXPORT int spx_batch_efourier_fetch(SpxBatch *b, float *efourier, float *fcfo);
 
 /* results of reconstruction `rec` (rec < 0: all) */
 spimage_EXPORT int spx_batch_get_model(SpxBatch *b, int rec, float *model_complex);

@@ -244,14 +244,31 @@ int spx_batch_iterate(SpxBatch* b, int iterations) {
   return run_phasing(b, iterations);
 }
 
-int spx_batch_efourier(SpxBatch* b, float* ef, float* fcfo) {
-  if (!b || !ef) return fail(-1, "bad batch arguments");
-  std::vector<double> s((size_t)4 * b->nrec);
-  if (int rc = b->e->efourier(s.data(), nullptr)) return rc;
+static void efourier_of_sums(const SpxBatch* b, const double* s, float* ef, float* fcfo) {
   for (int r = 0; r < b->nrec; ++r) {
     ef[r] = (float)sqrt(s[4 * r] / s[4 * r + 1]);
     if (fcfo) fcfo[r] = (float)(s[4 * r + 2] / s[4 * r + 3]);
   }
+}
+
+int spx_batch_efourier(SpxBatch* b, float* ef, float* fcfo) {
+  if (!b || !ef) return fail(-1, "bad batch arguments");
+  std::vector<double> s((size_t)4 * b->nrec);
+  if (int rc = b->e->efourier(s.data(), nullptr)) return rc;
+  efourier_of_sums(b, s.data(), ef, fcfo);
+  return 0;
+}
+
+int spx_batch_efourier_enqueue(SpxBatch* b) {
+  if (!b) return fail(-1, "bad batch arguments");
+  return b->e->efourier_enqueue();
+}
+
+int spx_batch_efourier_fetch(SpxBatch* b, float* ef, float* fcfo) {
+  if (!b || !ef) return fail(-1, "bad batch arguments");
+  std::vector<double> s((size_t)4 * b->nrec);
+  if (int rc = b->e->efourier_fetch(s.data())) return rc;
+  efourier_of_sums(b, s.data(), ef, fcfo);
   return 0;
 }
 

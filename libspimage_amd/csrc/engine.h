@@ -122,6 +122,10 @@ class Engine {
 
   // ---- metrics (reference src/phasing.c:803-893), nrec results each ----
   int efourier(double* out /* 4 per rec */, float2* fmodel_out /* device, may be null */);
+  // the same sums, enqueued behind the work on the engine stream into pinned
+  // host memory; efourier_fetch returns them once the stream has reached them
+  int efourier_enqueue();
+  int efourier_fetch(double* out /* 4 per rec */);
   int fmodel(float2* fmodel_out);
   int ereal(double* out /* 2 per rec */);
   int support_count(double* out /* 1 per rec */);
@@ -172,6 +176,8 @@ class Engine {
   uint8_t* flags = nullptr;
   double* partials = nullptr;
   double* red = nullptr;
+  double* red_host = nullptr;  // pinned, 4 per rec (efourier_enqueue)
+  bool red_host_pending = false;
   unsigned* hist = nullptr;   // kHistStride per rec
   unsigned* ctl = nullptr;    // kCtlStride per rec
   int* alg_rec = nullptr;     // optional per-rec tables

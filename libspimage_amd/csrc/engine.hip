@@ -867,6 +867,7 @@ Engine::~Engine() {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : templates) (void)hipFree(e.second);
+  if (red_host) (void)hipHostFree(red_host);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (s) (void)hipStreamDestroy(s);
@@ -1260,6 +1261,32 @@ int Engine::efourier(double* out, float2* fout) {
   enqueue_fourier(sh, kColEFourier, W, S1, ca, s, sh.t4);
   SPX_HIP(hipGetLastError());
   return reduce_per_rec(last_axis_tiles_per_rec(sh), 4, out);
+}
+
+int Engine::efourier_enqueue() {
+  if (int r = ensure_R()) return r;
+  if (!red_host) SPX_HIP(hipHostMalloc((void**)&red_host, sizeof(double) * 4 * (size_t)nrec, hipHostMallocDefault));
+  ColArgs ca{};
+  ca.ampx = ampx;
+  ca.partials = partials;
+  enqueue_fourier(sh, kColEFourier, W, S1, ca, s, sh.t4);
+  SPX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_rec, dim3(nrec), dim3(256), 0, s, partials, last_axis_tiles_per_rec(sh), 4, red);
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipMemcpyAsync(red_host, red, sizeof(double) * 4 * (size_t)nrec, hipMemcpyDeviceToHost, s));
+  red_host_pending = true;
+  return 0;
+}
+
+int Engine::efourier_fetch(double* out) {
+  if (!red_host_pending) {
+    set_error("no Fourier error was enqueued");
+    return -1;
+  }
+  SPX_HIP(hipStreamSynchronize(s));
+  memcpy(out, red_host, sizeof(double) * 4 * (size_t)nrec);
+  red_host_pending = false;
+  return 0;
 }
 
 int Engine::fmodel(float2* fout) {

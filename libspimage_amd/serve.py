@@ -15,8 +15,11 @@ Scheduling:
   result equals the same request run alone through sp_phaser_iterate;
 * continuous batching -- a batch advances in quanta of `quantum`
   iterations.  Between quanta finished reconstructions
-  leave and waiting requests of the class take their slots; when every slot
-  of a class is busy and requests wait, a further batch is created for them;
+  leave and waiting requests of the class take their slots; only when more
+  requests wait than the class has free slots (in running batches too,
+  which take requests at the end of their quantum) is a further batch
+  created, with at least `min_slots` slots, so that requests arriving one by
+  one share launches instead of each opening a batch of one;
 * admission by free device memory -- a batch is created only if its
   estimated footprint (spx_batch_estimate_bytes) fits into the free device
   memory minus a reserve; otherwise idle batches of other classes are
@@ -25,7 +28,12 @@ Scheduling:
   makes the pass kernels skip the work-groups of slots without a request;
 * every batch owns a HIP stream, so quanta of different classes overlap on
   the device; the host loop only polls (spx_batch_query) and never waits
-  for one class while another could be fed.
+  for one class while another could be fed: the Fourier errors of the
+  reconstructions that finish with a quantum are queued behind it
+  (spx_batch_efourier_enqueue) and read once it is done;
+* batches live until memory is needed: device memory is freed only when an
+  admission does not fit (freeing synchronises the whole device, which
+  would stall every other class), unless `idle_release` seconds are given.
 
 Request iteration counts must be multiples of `quantum`.
 """
@@ -90,8 +98,9 @@ class _Batch:
 
 class Server:
     def __init__(self, quantum: int = 50, max_pixels_per_batch: int = 1 << 24, reserve_bytes: int = 8 << 30,
-                 keep_models: bool = False, idle_release: float = 1.0):
+                 keep_models: bool = False, idle_release: float | None = None, min_slots: int = 4):
         self.quantum = quantum
+        self.min_slots = max(1, min_slots)
         self.max_pixels = max_pixels_per_batch
         self.reserve = reserve_bytes
         self.keep_models = keep_models
@@ -110,7 +119,7 @@ class Server:
         npix = int(np.prod(shape))
         cap_max = max(1, self.max_pixels // npix)
         cap = 1
-        while cap < waiting and cap < cap_max:
+        while (cap < waiting or cap < self.min_slots) and cap < cap_max:
             cap *= 2
         cap = min(cap, cap_max)
         free, _ = sp.device_memory()
@@ -181,7 +190,7 @@ class Server:
         fin = [r for r, s in enumerate(bt.slots) if s is not None and s.done >= s.req.iterations]
         if not fin:
             return
-        ef, fc = bt.batch.efourier()
+        ef, fc = bt.batch.efourier_fetch()
         t = now_fn()
         for r in fin:
             s = bt.slots[r]
@@ -229,7 +238,11 @@ class Server:
                     if not bt.busy and self._fill(bt, q, now()):
                         progressed = True
                 while q:
-                    bt = self._new_batch(shape, len(q))
+                    # free slots of running batches take requests when their quantum ends
+                    free = sum(s is None for bt in self.classes.get(shape, []) for s in bt.slots)
+                    if free >= len(q):
+                        break
+                    bt = self._new_batch(shape, len(q) - free)
                     if bt is None:
                         break
                     self._fill(bt, q, now())
@@ -239,6 +252,8 @@ class Server:
                 for bt in blist:
                     if not bt.busy and any(s is not None for s in bt.slots):
                         bt.batch.iterate(self.quantum)
+                        if any(s is not None and s.done + self.quantum >= s.req.iterations for s in bt.slots):
+                            bt.batch.efourier_enqueue()
                         bt.busy = True
                         progressed = True
             if self.idle_release is not None:

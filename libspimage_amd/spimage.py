@@ -276,6 +276,8 @@ _bind("spx_batch_use_graphs", C.c_int, PBatch, C.c_int)
 _bind("spx_batch_iterate", C.c_int, PBatch, C.c_int)
 _bind("spx_batch_iteration", C.c_int, PBatch)
 _bind("spx_batch_efourier", C.c_int, PBatch, PF, PF)
+_bind("spx_batch_efourier_enqueue", C.c_int, PBatch)
+_bind("spx_batch_efourier_fetch", C.c_int, PBatch, PF, PF)
 _bind("spx_batch_ereal", C.c_int, PBatch, PF)
 _bind("spx_batch_support_fraction", C.c_int, PBatch, PF)
 _bind("spx_batch_get_model", C.c_int, PBatch, C.c_int, PF)
@@ -378,7 +380,7 @@ class Batch:
     def set_rec_algorithms(self, algs=None, betas=None):
         a = None if algs is None else np.ascontiguousarray(algs, np.int32)
         b = None if betas is None else np.ascontiguousarray(betas, np.float32)
-        self._keep += [a, b]
+        # copied to the device before the call returns: nothing to keep alive
         self._check(spx_batch_set_rec_algorithms(self.h, None if a is None else _iptr(a),
                                                  None if b is None else _fptr(b)), "set_rec_algorithms")
 
@@ -400,6 +402,17 @@ class Batch:
         e = np.empty(self.nrec, np.float32)
         f = np.empty(self.nrec, np.float32)
         self._check(spx_batch_efourier(self.h, _fptr(e), _fptr(f)), "efourier")
+        return e, f
+
+    def efourier_enqueue(self):
+        """Queues the Fourier errors behind the queued work; efourier_fetch
+        returns them (without waiting once done() is true)."""
+        self._check(spx_batch_efourier_enqueue(self.h), "efourier_enqueue")
+
+    def efourier_fetch(self):
+        e = np.empty(self.nrec, np.float32)
+        f = np.empty(self.nrec, np.float32)
+        self._check(spx_batch_efourier_fetch(self.h, _fptr(e), _fptr(f)), "efourier_fetch")
         return e, f
 
     def ereal(self):
@@ -452,7 +465,6 @@ class Batch:
     def set_active(self, active=None):
         """nrec flags (0 = idle slot, left unchanged by iterations), or None for all."""
         a = None if active is None else np.ascontiguousarray(active, np.int32)
-        self._keep.append(a)
         self._check(spx_batch_set_active(self.h, None if a is None else _iptr(a)), "set_active")
 
     def init_model_rec(self, rec, flags, seed):

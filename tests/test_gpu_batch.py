@@ -147,3 +147,32 @@ def test_idle_reconstructions_are_left_unchanged(sp, shape):
     m, s, e = run_single(sp, pats[1], "hio", 3)
     assert np.array_equal(b.model(1), m)
     b.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (16, 16, 16)])
+def test_queued_efourier_equals_blocking(sp, shape):
+    """spx_batch_efourier_enqueue / _fetch: the errors queued behind the
+    iterations equal the blocking spx_batch_efourier's bit for bit, the
+    queued call leaves the iterates alone, and a fetch needs an enqueue."""
+    pats = patterns(shape, 3, seed=95)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.iterate(9)
+    b.efourier_enqueue()
+    while not b.done():
+        pass
+    ef_q, fc_q = b.efourier_fetch()
+    ef, fc = b.efourier()
+    assert np.array_equal(ef_q, ef) and np.array_equal(fc_q, fc)
+    with pytest.raises(RuntimeError):
+        b.efourier_fetch()
+    # queued between iterations, the trajectory is unchanged
+    b.iterate(4)
+    b.efourier_enqueue()
+    b.iterate(3)
+    b.efourier_fetch()
+    c = make_batch(sp, pats, hio)
+    c.iterate(16)
+    assert np.array_equal(b.model(), c.model())
+    b.close()
+    c.close()
