@@ -54,6 +54,7 @@ struct Shape {
   const LengthEntry *ex, *ey, *ez;
   LineGeom gy, gz;
   bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
+  bool colw;     // 2-D row-major: the projection pass runs with the wider work-groups
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);

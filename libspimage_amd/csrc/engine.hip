@@ -211,10 +211,15 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
 // whatever layout each uses; the layout is chosen per length pair, for
 // single reconstructions and batches apart (in a batch whole reconstructions
 // stay on one XCD, which favours the tiled layout).  Returns true for the
-// tiled layout.
-static std::map<std::tuple<LengthEntry*, LengthEntry*, bool>, bool> tune_layout;
+// tiled layout.  *wide: in the row-major layout, the projection pass runs
+// with the wider work-groups (LengthOps::col_project_wide), also timed here
+// with the chosen variants; SPX_COL_WIDE=0/1 forces the choice.
+struct PairChoice {
+  bool t4, wide;
+};
+static std::map<std::tuple<LengthEntry*, LengthEntry*, bool>, PairChoice> tune_layout;
 
-static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_shape) {
+static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_shape, bool* wide) {
   std::lock_guard<std::mutex> lock(tune_mu);
   const bool need_row = !(tune_done[ex] & 1), need_col = !(tune_done[ey] & 2);
   tune_done[ex] |= 1;
@@ -230,12 +235,19 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
     if (yes4 && atoi(yes4) == 1) return possible;
     return t4 && possible;
   };
+  const char* wenv = getenv("SPX_COL_WIDE");
+  auto finish = [&](PairChoice pc) {
+    const bool t4 = decided(pc.t4);
+    bool w = wenv ? atoi(wenv) == 1 : pc.wide;
+    *wide = !t4 && w && ey->var[ey->col_choice].wide;
+    return t4;
+  };
   const auto key = std::make_tuple(ex, ey, nrec > 1);
   auto it = tune_layout.find(key);
-  if (it != tune_layout.end()) return decided(it->second);
+  if (it != tune_layout.end()) return finish(it->second);
   if (tuning_off()) {
-    tune_layout[key] = false;
-    return decided(false);
+    tune_layout[key] = PairChoice{false, false};
+    return finish(tune_layout[key]);
   }
   const int nx = ex->L;
   const long N = (long)nx * ny;
@@ -258,18 +270,24 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
   }
   float best = 1e30f;
   int br = ex->row_choice, bc = ey->col_choice, blay = 0;
-  for (int lay = 0; ok && lay < 2; ++lay) {
-    const bool t4 = lay == 1;
+  bool bwide = false;
+  // lay 0: row-major, 1: tiled, 2: row-major with the wide projection pass
+  // (only for the best row-major variants, once lay 0 and 1 are timed)
+  int r2 = -1, c2 = -1;
+  float best_rm = 1e30f;
+  for (int lay = 0; ok && lay < 3; ++lay) {
+    const bool t4 = lay == 1, wd = lay == 2;
     if (t4 && !t4_shape) continue;
+    if (wd && (r2 < 0 || !ey->var[c2].wide)) continue;
     LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
     if (t4) {
       g.tshift = 2;
       g.tstride = 4L * ny;
     }
     for (int r = 0; r < ex->nvar; ++r) {
-      if ((!need_row && r != ex->row_choice) || (t4 && !ex->var[r].t4)) continue;
+      if ((!need_row && r != ex->row_choice) || (t4 && !ex->var[r].t4) || (wd && r != r2)) continue;
       for (int c = 0; c < ey->nvar; ++c) {
-        if (!need_col && c != ey->col_choice) continue;
+        if ((!need_col && c != ey->col_choice) || (wd && c != c2)) continue;
         ColArgs ca{};
         ca.in = w;
         ca.W = w;
@@ -294,19 +312,26 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
             [&] {
               u.gcur = ga;
               u.gnew = gb;
-              vc.col(kColProject, ca, s);
+              if (wd) vc.col_project_wide(ca, s);
+              else vc.col(kColProject, ca, s);
               vr.update(u, false, t4, s);
               std::swap(ga, gb);
             },
             20, 30);
         if (getenv("SPX_TUNE_LOG"))
-          fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr, t4 ? "tiled" : "row-major",
-                  vr.E, vc.E, ms * 1000.f);
+          fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
+                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), vr.E, vc.E, ms * 1000.f);
+        if (lay == 0 && ms < best_rm) {
+          best_rm = ms;
+          r2 = r;
+          c2 = c;
+        }
         if (ms < best) {
           best = ms;
           br = r;
           bc = c;
-          blay = lay;
+          blay = t4 ? 1 : 0;
+          bwide = wd;
         }
       }
     }
@@ -318,8 +343,8 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (s) (void)hipStreamDestroy(s);
-  tune_layout[key] = blay == 1;
-  return decided(blay == 1);
+  tune_layout[key] = PairChoice{blay == 1, bwide};
+  return finish(tune_layout[key]);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,7 +719,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt.tshift = 2;
   sh->gyt.tstride = 4L * ny;
   if (found[0] && found[1] && !found[2]) {
-    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0);
+    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->colw);
   } else {
     if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
     if (found[1]) tune_length(found[1], 0, &sh->gy);
@@ -782,7 +807,8 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     a.W = out;
     a.geo = tiled ? sh.gyt : sh.gy;
     a.outer_per_rec = 1;
-    sh.ey->col(mode, a, s);
+    if (mode == kColProject && !tiled && sh.colw) sh.ey->col_project_wide(a, s);
+    else sh.ey->col(mode, a, s);
   }
 }
 

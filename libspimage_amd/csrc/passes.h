@@ -530,7 +530,12 @@ __global__ __launch_bounds__((RowGeo<L, E, true>::NT)) void k_row_fwd_t4(RowArgs
 // ---------------------------------------------------------------------------
 // Column kernels: C adjacent lines (contiguous in memory) per work-group.
 // ---------------------------------------------------------------------------
-template <int L, int E>
+// WIDE: at least 8 adjacent lines (64-byte segments).  In the row-major
+// layout 32-byte segments waste much of each 128-byte line fetched, so a
+// batch with enough column groups to fill the chip runs its projection pass
+// wide; a single reconstruction keeps 4 lines per work-group and twice the
+// work-groups (the tuner times both).
+template <int L, int E, bool WIDE = false>
 struct ColShape {
   static constexpr int T = PlanOf<L, E>::T;
   // C >= 4 adjacent lines (32-byte segments) and about 256 threads, within
@@ -539,17 +544,18 @@ struct ColShape {
   static constexpr int Cthr = (1024 / T) >= 1 ? (1024 / T) : 1;
   static constexpr int NBUF = SingleBuf<L>::buffers;
   static constexpr int Cmem = (kLdsBytes / (NBUF * LS * 8)) >= 1 ? (kLdsBytes / (NBUF * LS * 8)) : 1;
-  static constexpr int Cw = (256 / T) > 4 ? (256 / T) : 4;
+  static constexpr int Cmin = WIDE ? 8 : 4;
+  static constexpr int Cw = (256 / T) > Cmin ? (256 / T) : Cmin;
   static constexpr int C1 = Cw < 64 ? Cw : 64;
   static constexpr int C2 = C1 < Cthr ? C1 : Cthr;
   static constexpr int C = C2 < Cmem ? C2 : Cmem;
   static constexpr int NT = T * C;
 };
 
-template <int L, int E, int MODE>
-__global__ __launch_bounds__((ColShape<L, E>::NT)) void k_col(ColArgs a) {
+template <int L, int E, int MODE, bool WIDE = false>
+__global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
   using PL = PlanOf<L, E>;
-  constexpr int T = ColShape<L, E>::T, C = ColShape<L, E>::C, NT = ColShape<L, E>::NT;
+  constexpr int T = ColShape<L, E, WIDE>::T, C = ColShape<L, E, WIDE>::C, NT = ColShape<L, E, WIDE>::NT;
   constexpr int NR = PL::NREG;
   using View = LdsView<L, C, true>;
   __shared__ float2 shA[View::SIZE];
@@ -730,6 +736,20 @@ struct LengthOps {
       case kColBlur: hipLaunchKernelGGL((k_col<L, E, kColBlur>), grid, blk, 0, s, a); break;
     }
   }
+  // the projection pass with at least 8 lines per work-group, where that
+  // differs from col()'s work-group
+  static constexpr bool wide = ColShape<L, E, true>::C > ColShape<L, E>::C;
+  static int colw_blocks(const LineGeom& g) {
+    return ((g.ncontig + ColShape<L, E, true>::C - 1) / ColShape<L, E, true>::C) * g.nouter;
+  }
+  static void col_project_wide(const ColArgs& a0, hipStream_t s) {
+    if constexpr (wide) {
+      ColArgs a = a0;
+      a.tw = tw();
+      hipLaunchKernelGGL((k_col<L, E, kColProject, true>), dim3(colw_blocks(a.geo)), dim3(ColShape<L, E, true>::NT), 0,
+                         s, a);
+    }
+  }
 };
 
 // One compiled variant (plan with E elements per thread at the edges).
@@ -741,12 +761,21 @@ struct Variant {
   void (*row_t4)(const RowArgs&, hipStream_t);
   void (*col)(int, const ColArgs&, hipStream_t);
   int (*col_blocks)(const LineGeom&);
+  bool wide;  // has a wider projection pass (col_project_wide)
+  void (*col_project_wide)(const ColArgs&, hipStream_t);
 };
 
 template <int L, int E>
 constexpr Variant variant() {
-  return Variant{E, LengthOps<L, E>::t4, &LengthOps<L, E>::row, &LengthOps<L, E>::update,
-                 &LengthOps<L, E>::row_t4, &LengthOps<L, E>::col, &LengthOps<L, E>::col_blocks};
+  return Variant{E,
+                 LengthOps<L, E>::t4,
+                 &LengthOps<L, E>::row,
+                 &LengthOps<L, E>::update,
+                 &LengthOps<L, E>::row_t4,
+                 &LengthOps<L, E>::col,
+                 &LengthOps<L, E>::col_blocks,
+                 LengthOps<L, E>::wide,
+                 &LengthOps<L, E>::col_project_wide};
 }
 
 // Dispatch table entry for one length: its compiled variants and the ones in
@@ -763,6 +792,7 @@ struct LengthEntry {
   }
   void row_t4(const RowArgs& a, hipStream_t s) const { var[row_choice].row_t4(a, s); }
   void col(int mode, const ColArgs& a, hipStream_t s) const { var[col_choice].col(mode, a, s); }
+  void col_project_wide(const ColArgs& a, hipStream_t s) const { var[col_choice].col_project_wide(a, s); }
   int col_blocks(const LineGeom& g) const { return var[col_choice].col_blocks(g); }
 };
 
