@@ -35,10 +35,10 @@ spimage_EXPORT void *spx_host_alloc(unsigned long long bytes);
 spimage_EXPORT void spx_host_free(void *p);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
-/* Elements per thread (4, 8 or 16) of the transform variant the tuner chose
- * for lines of length L in row passes and in column passes.  One choice per
- * length serves both layouts of the Fourier-space arrays; `tiled` is kept
- * for callers that ask per layout. */
+/* Elements per thread (4, 8 or 16) of the transform variants the tuner chose
+ * for L x L arrays in row passes and in column passes (tuning the shape if
+ * no engine has used it yet).  One choice per shape serves both layouts of
+ * the Fourier-space arrays; `tiled` is kept for callers that ask per layout. */
 spimage_EXPORT int spx_fft_variant(int L, int tiled, int *row_elements, int *col_elements);
 
 /* 1 when the phaser's engine keeps its Fourier-space work arrays (the

@@ -1188,14 +1188,15 @@ void spx_host_free(void* p) {
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
 int spx_fft_variant(int L, int tiled, int* row_e, int* col_e) {
-  const spx::LengthEntry* e = spx::find_length(L);
-  if (!e) {
+  if (!spx::find_length(L)) {
     spx::set_error("length %d is not supported", L);
     return -11;
   }
-  (void)tiled;  // one choice per length serves both layouts
-  if (row_e) *row_e = e->var[e->row_choice].E;
-  if (col_e) *col_e = e->var[e->col_choice].E;
+  (void)tiled;  // one choice per shape serves both layouts
+  int e[3];
+  if (int rc = spx::shape_variant_elements(L, L, 1, e)) return rc;
+  if (row_e) *row_e = e[0];
+  if (col_e) *col_e = e[1];
   return 0;
 }
 

@@ -35,11 +35,10 @@ const char* last_error();
   } while (0)
 
 bool device_available();
-// Chooses, once per process and length, the plan variant (elements per
-// thread) of row passes and of column passes by timing every compiled
-// variant on the given geometry.  SPX_FFT_E=4|8|16 forces a variant and
-// SPX_NO_TUNE=1 keeps the default (8).
-void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo);
+// make_shape chooses, once per process and array shape, the plan variant
+// (elements per thread) of each axis by timing the compiled variants on the
+// shape's geometry.  SPX_FFT_E=4|8|16 forces a variant and SPX_NO_TUNE=1
+// keeps the default (8).
 
 struct IterParams {
   int alg;            // SpPhasingAlgorithmType
@@ -51,13 +50,17 @@ struct IterParams {
 // Lengths and line geometry of one (nz, ny, nx) shape stacked nrec times.
 struct Shape {
   int nx, ny, nz, nrec;
-  const LengthEntry *ex, *ey, *ez;
+  AxisOps ex, ey, ez;  // per-axis passes with the variants chosen for this shape
   LineGeom gy, gz;
   bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
   bool colw;     // 2-D row-major: the projection pass runs with the wider work-groups
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
+// elements per thread (4, 8 or 16; 0 for a trivial axis) of the x, y and z
+// transform variants chosen for arrays of shape (nz, ny, nx), tuning the
+// shape on its first use
+int shape_variant_elements(int nx, int ny, int nz, int* e);
 // unnormalised transform of every reconstruction of `sh` (dir < 0 forward,
 // e^{-2 pi i k.n/N}); in == out is allowed
 void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s);

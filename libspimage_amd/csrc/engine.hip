@@ -113,8 +113,17 @@ __global__ void k_tune_fill(float2* w, float2* g0, float2* g1, float* amp, uint8
 }
 
 static std::mutex tune_mu;
-// per length: bit 1 row variant chosen, bit 2 column variant chosen
-static std::map<LengthEntry*, int> tune_done;
+
+// Variant choices are made per array shape (nx, ny, nz), once: the first
+// engine, batch or transform of that shape tunes them and every later one
+// of the same shape, whatever its number of reconstructions or layout, uses
+// the same variants, so a reconstruction computes the same bits alone and in
+// a batch.  Shapes do not share choices: a 2-D 256 x 256 request and a
+// 256^3 volume each get the variants that suit their own passes.
+struct ShapeVariants {
+  int v[3];
+};
+static std::map<std::tuple<int, int, int>, ShapeVariants> shape_variants;
 
 // variant with E elements per thread (forced by SPX_FFT_E, else 8) among
 // those usable in the layout, else the first usable one
@@ -136,13 +145,10 @@ static bool tuning_off() {
   return (force && atoi(force) > 0) || (notune && atoi(notune) == 1);
 }
 
-// Single-pass tuning for lengths outside a 2-D pair (3-D axes): times the
-// pass's hot kernel on zero-filled scratch of the real geometry.
-void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
-  std::lock_guard<std::mutex> lock(tune_mu);
-  const int bit = col_geo ? 2 : 1;
-  if (tune_done[e] & bit) return;
-  tune_done[e] |= bit;
+// Single-pass tuning of one axis outside a 2-D pair (1-D and 3-D shapes):
+// times the pass's hot kernel on zero-filled scratch of the real geometry
+// and returns the variant index.  Caller holds tune_mu.
+static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo) {
   int choice = default_choice(e, false);
   if (e->nvar > 1 && !tuning_off()) {
     const long L = e->L;
@@ -185,6 +191,9 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
           u.nsize = 1.f;
           ms = time_launches(s, [&] { v.update(u, false, false, s); });
         }
+        if (getenv("SPX_TUNE_LOG"))
+          fprintf(stderr, "tune length %d (%s, %d lines): E=%d  %.2f us\n", (int)L, col_geo ? "column" : "row",
+                  col_geo ? col_geo->ncontig * col_geo->nouter : lines, v.E, ms * 1000.f);
         if (ms < best) {
           best = ms;
           choice = i;
@@ -197,8 +206,7 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
       if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
   }
-  if (col_geo) e->col_choice = choice;
-  else e->row_choice = choice;
+  return choice;
 }
 
 // Joint tuning of a 2-D shape's row (x) and column (y) passes, timed as one
@@ -206,43 +214,46 @@ void tune_length(LengthEntry* e, int lines, const LineGeom* col_geo) {
 // variant's cost depends on where (which XCD's cache) the previous pass left
 // the data.  Both layouts of the Fourier-space arrays are timed (row-major,
 // and 4-column tiled when nx % 4 == 0 and the row variant has a tiled
-// kernel).  The variants are chosen once per length and serve both layouts,
-// so that a reconstruction computes the same bits alone and in a batch
-// whatever layout each uses; the layout is chosen per length pair, for
-// single reconstructions and batches apart (in a batch whole reconstructions
+// kernel).  The variants (*vx, *vy) are chosen at the shape's first use and
+// serve both layouts; the layout is chosen per shape for single
+// reconstructions and for batches apart (in a batch whole reconstructions
 // stay on one XCD, which favours the tiled layout).  Returns true for the
 // tiled layout.  *wide: in the row-major layout, the projection pass runs
 // with the wider work-groups (LengthOps::col_project_wide), also timed here
-// with the chosen variants; SPX_COL_WIDE=0/1 forces the choice.
+// with the chosen variants; SPX_COL_WIDE=0/1 forces the choice.  Caller
+// holds tune_mu.
 struct PairChoice {
   bool t4, wide;
 };
-static std::map<std::tuple<LengthEntry*, LengthEntry*, bool>, PairChoice> tune_layout;
+static std::map<std::tuple<int, int, bool>, PairChoice> tune_layout;
 
-static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t4_shape, bool* wide) {
-  std::lock_guard<std::mutex> lock(tune_mu);
-  const bool need_row = !(tune_done[ex] & 1), need_col = !(tune_done[ey] & 2);
-  tune_done[ex] |= 1;
-  tune_done[ey] |= 2;
-  if (need_row) ex->row_choice = default_choice(ex, false);
-  if (need_col) ey->col_choice = default_choice(ey, false);
+static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
+                      bool* wide) {
+  const auto dims = std::make_tuple(ex->L, ny, 1);
+  auto known = shape_variants.find(dims);
+  const bool need_row = known == shape_variants.end(), need_col = need_row;
+  int row_choice = need_row ? default_choice(ex, false) : known->second.v[0];
+  int col_choice = need_col ? default_choice(ey, false) : known->second.v[1];
   // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows it)
   const char* not4 = getenv("SPX_NO_T4");
   const char* yes4 = getenv("SPX_T4");
   auto decided = [&](bool t4) {
-    const bool possible = t4_shape && ex->var[ex->row_choice].t4;
+    const bool possible = t4_shape && ex->var[row_choice].t4;
     if (not4 && atoi(not4) == 1) return false;
     if (yes4 && atoi(yes4) == 1) return possible;
     return t4 && possible;
   };
   const char* wenv = getenv("SPX_COL_WIDE");
   auto finish = [&](PairChoice pc) {
+    if (need_row) shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
+    *vx = row_choice;
+    *vy = col_choice;
     const bool t4 = decided(pc.t4);
     bool w = wenv ? atoi(wenv) == 1 : pc.wide;
-    *wide = !t4 && w && ey->var[ey->col_choice].wide;
+    *wide = !t4 && w && ey->var[col_choice].wide;
     return t4;
   };
-  const auto key = std::make_tuple(ex, ey, nrec > 1);
+  const auto key = std::make_tuple(ex->L, ny, nrec > 1);
   auto it = tune_layout.find(key);
   if (it != tune_layout.end()) return finish(it->second);
   if (tuning_off()) {
@@ -269,7 +280,7 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
     ok = hipGetLastError() == hipSuccess;
   }
   float best = 1e30f;
-  int br = ex->row_choice, bc = ey->col_choice, blay = 0;
+  int br = row_choice, bc = col_choice, blay = 0;
   bool bwide = false;
   // lay 0: row-major, 1: tiled, 2: row-major with the wide projection pass
   // (only for the best row-major variants, once lay 0 and 1 are timed)
@@ -285,9 +296,9 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
       g.tstride = 4L * ny;
     }
     for (int r = 0; r < ex->nvar; ++r) {
-      if ((!need_row && r != ex->row_choice) || (t4 && !ex->var[r].t4) || (wd && r != r2)) continue;
+      if ((!need_row && r != row_choice) || (t4 && !ex->var[r].t4) || (wd && r != r2)) continue;
       for (int c = 0; c < ey->nvar; ++c) {
-        if ((!need_col && c != ey->col_choice) || (wd && c != c2)) continue;
+        if ((!need_col && c != col_choice) || (wd && c != c2)) continue;
         ColArgs ca{};
         ca.in = w;
         ca.W = w;
@@ -336,8 +347,8 @@ static bool tune_pair(LengthEntry* ex, LengthEntry* ey, int ny, int nrec, bool t
       }
     }
   }
-  ex->row_choice = br;
-  ey->col_choice = bc;
+  row_choice = br;
+  col_choice = bc;
   (void)hipStreamSynchronize(s);
   void* bufs[] = {w, g0, g1, amp, fl};
   for (void* p : bufs)
@@ -698,8 +709,8 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
     return -11;
   }
   const int dims[3] = {nx, ny, nz};
-  const LengthEntry** e[3] = {&sh->ex, &sh->ey, &sh->ez};
-  LengthEntry* found[3] = {nullptr, nullptr, nullptr};
+  AxisOps* ax[3] = {&sh->ex, &sh->ey, &sh->ez};
+  const LengthEntry* found[3] = {nullptr, nullptr, nullptr};
   for (int a = 0; a < 3; ++a) {
     if (dims[a] == 1) continue;
     found[a] = find_length(dims[a]);
@@ -707,7 +718,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
       set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
       return -11;
     }
-    *e[a] = found[a];
+    ax[a]->e = found[a];
   }
   // y lines: (rec, z) outer blocks of nx adjacent lines; z lines: rec outer
   // blocks of nx*ny adjacent lines
@@ -718,13 +729,29 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt = LineGeom{nx, nrec, (long)nx * ny, 4L};
   sh->gyt.tshift = 2;
   sh->gyt.tstride = 4L * ny;
+  std::lock_guard<std::mutex> lock(tune_mu);
   if (found[0] && found[1] && !found[2]) {
-    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->colw);
-  } else {
-    if (found[0]) tune_length(found[0], ny * nz * nrec, nullptr);
-    if (found[1]) tune_length(found[1], 0, &sh->gy);
-    if (found[2]) tune_length(found[2], 0, &sh->gz);
+    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw);
+    return 0;
   }
+  const auto key = std::make_tuple(nx, ny, nz);
+  auto known = shape_variants.find(key);
+  if (known != shape_variants.end()) {
+    for (int a = 0; a < 3; ++a) ax[a]->v = known->second.v[a];
+    return 0;
+  }
+  if (found[0]) sh->ex.v = tune_length(found[0], ny * nz * nrec, nullptr);
+  if (found[1]) sh->ey.v = tune_length(found[1], 0, &sh->gy);
+  if (found[2]) sh->ez.v = tune_length(found[2], 0, &sh->gz);
+  shape_variants[key] = ShapeVariants{{sh->ex.v, sh->ey.v, sh->ez.v}};
+  return 0;
+}
+
+int shape_variant_elements(int nx, int ny, int nz, int* e) {
+  Shape sh;
+  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
+  const AxisOps* ax[3] = {&sh.ex, &sh.ey, &sh.ez};
+  for (int a = 0; a < 3; ++a) e[a] = *ax[a] ? (*ax[a])->var().E : 0;
   return 0;
 }
 

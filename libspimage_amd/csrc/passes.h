@@ -778,26 +778,31 @@ constexpr Variant variant() {
                  &LengthOps<L, E>::col_project_wide};
 }
 
-// Dispatch table entry for one length: its compiled variants and the ones in
-// use (chosen by the tuner) for row and column passes, in either layout.
+// Dispatch table entry for one length: its compiled variants.
 struct LengthEntry {
   int L;
   int nvar;
   Variant var[3];
-  int row_choice;
-  int col_choice;
-  void row(int mode, const RowArgs& a, hipStream_t s) const { var[row_choice].row(mode, a, s); }
-  void update(const UpdateArgs& a, bool dm, bool tiled, hipStream_t s) const {
-    var[row_choice].update(a, dm, tiled, s);
-  }
-  void row_t4(const RowArgs& a, hipStream_t s) const { var[row_choice].row_t4(a, s); }
-  void col(int mode, const ColArgs& a, hipStream_t s) const { var[col_choice].col(mode, a, s); }
-  void col_project_wide(const ColArgs& a, hipStream_t s) const { var[col_choice].col_project_wide(a, s); }
-  int col_blocks(const LineGeom& g) const { return var[col_choice].col_blocks(g); }
 };
 
-// Registry implemented by the instantiation units (mutable: the tuner sets
-// the choices).
+// The passes of one axis of a shape: the length's entry and the variant the
+// tuner chose for that shape (the value behaves like a pointer: `ax->row(..)`,
+// and false when the axis is trivial).
+struct AxisOps {
+  const LengthEntry* e = nullptr;
+  int v = 0;
+  explicit operator bool() const { return e != nullptr; }
+  const AxisOps* operator->() const { return this; }
+  const Variant& var() const { return e->var[v]; }
+  void row(int mode, const RowArgs& a, hipStream_t s) const { var().row(mode, a, s); }
+  void update(const UpdateArgs& a, bool dm, bool tiled, hipStream_t s) const { var().update(a, dm, tiled, s); }
+  void row_t4(const RowArgs& a, hipStream_t s) const { var().row_t4(a, s); }
+  void col(int mode, const ColArgs& a, hipStream_t s) const { var().col(mode, a, s); }
+  void col_project_wide(const ColArgs& a, hipStream_t s) const { var().col_project_wide(a, s); }
+  int col_blocks(const LineGeom& g) const { return var().col_blocks(g); }
+};
+
+// Registry implemented by the instantiation units.
 LengthEntry* find_length(int L);
 
 }  // namespace spx

@@ -242,8 +242,8 @@ _bind("spx_fft_variant", C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTE
 
 
 def fft_variant(L: int, tiled: bool = False) -> tuple[int, int]:
-    """(row, column) elements-per-thread of the tuned transform of length L
-    in the row-major or the tiled layout."""
+    """(row, column) elements-per-thread of the tuned transforms of L x L
+    arrays (in either layout)."""
     r, c = C.c_int(), C.c_int()
     if spx_fft_variant(L, 1 if tiled else 0, C.byref(r), C.byref(c)) != 0:
         raise ValueError(last_error())
