@@ -15,8 +15,9 @@ pass, 37 N for the Fourier-space pass), the whole-iteration roofline, the
 1024^2 RAAR + shrinkwrap reconstruction (300 iterations, area support every
 20), the batched workload (128 patterns x 8 starts), the served workload
 (Poisson 200 req/s), the 256^3 DM + ER reconstruction, the Reconstructor
-front-end's reconstruct_loop, the PRTF over many reconstructions, and a CPU
-baseline: the oracle's restatement of the reference loop on one host core.
+front-end's reconstruct_loop, the PRTF over many reconstructions, the
+iteration time of 2-D shapes with mixed-radix lengths, and a CPU baseline:
+the oracle's restatement of the reference loop on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                     [--no-batched] [--no-serving] [--no-volume]
@@ -49,7 +50,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
-                    help="skip the many-reconstruction workloads (batched, Reconstructor, PRTF)")
+                    help="skip the many-reconstruction workloads (batched, Reconstructor, PRTF, mixed-radix shapes)")
     ap.add_argument("--no-serving", action="store_true", help="skip the served workload")
     ap.add_argument("--no-volume", action="store_true", help="skip the 256^3 reconstruction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -240,6 +241,35 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
 
 
+def mixed_radix_bench(sizes=(720, 750, 810, 900, 972, 1000, 768, 960), nrec=16, iters=100, seed=7):
+    """HIO iterations (device time, HIP events) of 2-D shapes whose lengths
+    are not powers of two, as the served mix has them: one reconstruction and
+    batches of `nrec`, per reconstruction-iteration and against the 78*N-byte
+    roofline.  The tuner picks layout, variants and the wide projection pass
+    per shape."""
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import workloads
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    out = {}
+    for n in sizes:
+        p = workloads.serving_pattern((n, n), seed)
+        row = {}
+        for cap in (1, nrec):
+            b = sp.Batch((n, n), cap)
+            for r in range(cap):
+                b.set_data(r, p.amplitudes, p.mask, p.support)
+            b.init_model(sp.SpModelRandomPhases, 1)
+            b.set_algorithm(hio)
+            b.time_iterations(10)
+            us = b.time_iterations(iters) / iters * 1e3
+            b.close()
+            row[f"x{cap}"] = {"us_per_iteration": us, "us_per_reconstruction_iteration": us / cap,
+                              "frac": 78.0 * n * n * cap / (us * 1e-6) / 8e12}
+        out[f"{n}x{n}"] = row
+    return {"workload": f"HIO beta 0.9, serving-mix patterns, 1 and {nrec} reconstructions per launch, "
+                        f"{iters} iterations timed", "shapes": out}
+
+
 def volume_bench(dm_its=200, er_its=100, period=20):
     """3-D reconstruction of the 256^3 ellipsoid pattern: difference map
     (beta 0.8, gamma1 = gamma2 = infinity -> Elser's values) for `dm_its`
@@ -411,16 +441,16 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
                               beta=0.9 if alg == "hio" else 0.87, arrival=t if realtime else 0.0)
                 for k, (t, s, alg, its, sd) in enumerate(trace)]
 
-    # warm-up: every shape through both paths (per-length tuning, twiddles).
-    # The tuner keys its layout choice on one reconstruction or several per
-    # launch, so the server sees each shape twice (a batch of two) and the
-    # plain loop once.
-    warm = [serve.Request(-1 - 2 * i - k, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
-            for i, s in enumerate(shapes) for k in range(2)]
-    plain = serve.PlainRunner()
-    for r in warm[::2]:
-        plain.run_one(r)
+    # warm-up: the tuner runs for every shape at every batch size the server
+    # may create (Server.prepare), then every shape goes once through both
+    # paths (twiddle tables, first launches)
     srv = serve.Server()
+    srv.prepare(shapes)
+    warm = [serve.Request(-1 - i, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
+            for i, s in enumerate(shapes)]
+    plain = serve.PlainRunner()
+    for r in warm:
+        plain.run_one(r)
     srv.run(warm, realtime=False)
     srv.close()
 
@@ -519,6 +549,7 @@ def main():
     volume = None if args.no_volume else volume_bench()
     reconstructor = None if args.no_batched else reconstructor_bench()
     prtf = None if args.no_batched else prtf_bench()
+    mixed = None if args.no_batched else mixed_radix_bench()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
@@ -543,6 +574,7 @@ def main():
         "volume_256": volume,
         "reconstructor": reconstructor,
         "prtf": prtf,
+        "mixed_radix_2d": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -35,6 +35,13 @@ spimage_EXPORT void *spx_host_alloc(unsigned long long bytes);
 spimage_EXPORT void spx_host_free(void *p);
 /* 1 when a line of length L can be transformed by the engine */
 spimage_EXPORT int spx_supported_length(int L);
+/* Tunes arrays of shape (nz, ny, nx) stacked nrec times as the first engine
+ * or batch of that shape and size class would (variants per shape; layout
+ * and column-pass width per class of nrec: 1, 2-3, 4-7, 8 or more), so that
+ * later engines and batches start without timing runs.  A server calls it
+ * for the shapes and batch sizes it expects.  0, or -11 for an unsupported
+ * shape, -20 without a device. */
+spimage_EXPORT int spx_tune_shape(int nx, int ny, int nz, int nrec);
 /* Elements per thread (4, 8 or 16) of the transform variants the tuner chose
  * for L x L arrays in row passes and in column passes (tuning the shape if
  * no engine has used it yet).  One choice per shape serves both layouts of

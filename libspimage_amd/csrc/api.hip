@@ -1187,6 +1187,15 @@ void spx_host_free(void* p) {
 
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
+int spx_tune_shape(int nx, int ny, int nz, int nrec) {
+  if (!spx::device_available()) {
+    spx::set_error("no HIP device");
+    return -20;
+  }
+  spx::Shape sh;
+  return spx::make_shape(nx, ny, nz, nrec, &sh);
+}
+
 int spx_fft_variant(int L, int tiled, int* row_e, int* col_e) {
   if (!spx::find_length(L)) {
     spx::set_error("length %d is not supported", L);

@@ -215,17 +215,21 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // the data.  Both layouts of the Fourier-space arrays are timed (row-major,
 // and 4-column tiled when nx % 4 == 0 and the row variant has a tiled
 // kernel).  The variants (*vx, *vy) are chosen at the shape's first use and
-// serve both layouts; the layout is chosen per shape for single
-// reconstructions and for batches apart (in a batch whole reconstructions
-// stay on one XCD, which favours the tiled layout).  Returns true for the
-// tiled layout.  *wide: in the row-major layout, the projection pass runs
+// serve both layouts; the layout is chosen per shape and per class of batch
+// size (1, 2-3, 4-7, 8 or more reconstructions: in a batch whole
+// reconstructions stay on one XCD, which favours the tiled layout, and more
+// of them fill the chip with wider work-groups).  Layouts and work-group
+// widths compute the same bits, so these choices never change a result.
+// Returns true for the tiled layout.  *wide: in the row-major layout, the projection pass runs
 // with the wider work-groups (LengthOps::col_project_wide), also timed here
 // with the chosen variants; SPX_COL_WIDE=0/1 forces the choice.  Caller
 // holds tune_mu.
 struct PairChoice {
   bool t4, wide;
 };
-static std::map<std::tuple<int, int, bool>, PairChoice> tune_layout;
+static std::map<std::tuple<int, int, int>, PairChoice> tune_layout;
+
+static int batch_class(int nrec) { return nrec <= 1 ? 0 : nrec < 4 ? 1 : nrec < 8 ? 2 : 3; }
 
 static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
                       bool* wide) {
@@ -253,7 +257,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     *wide = !t4 && w && ey->var[col_choice].wide;
     return t4;
   };
-  const auto key = std::make_tuple(ex->L, ny, nrec > 1);
+  const auto key = std::make_tuple(ex->L, ny, batch_class(nrec));
   auto it = tune_layout.find(key);
   if (it != tune_layout.end()) return finish(it->second);
   if (tuning_off()) {
@@ -262,8 +266,10 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   }
   const int nx = ex->L;
   const long N = (long)nx * ny;
-  // cap the scratch at 16 Mi elements (whole reconstructions)
-  int nr = nrec;
+  // the class of 8 or more reconstructions is timed with at least 16, the
+  // sizes it mostly serves; the scratch is capped at 16 Mi elements (whole
+  // reconstructions)
+  int nr = batch_class(nrec) == 3 && nrec < 16 ? 16 : nrec;
   if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
   if (nr < 1) nr = 1;
   const long n = (long)nr * N;

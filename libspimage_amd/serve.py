@@ -114,6 +114,19 @@ class Server:
         self._base = sp.sp_phasing_hio_alloc(self._beta_map, 0)
         self.batches_created = 0
 
+    def prepare(self, shapes) -> None:
+        """Runs the tuner for every shape at each batch size this server may
+        create (and for one reconstruction), so that no timing run of the
+        tuner falls into serving."""
+        for shape in shapes:
+            cap_max = max(1, self.max_pixels // int(np.prod(shape)))
+            cap = 1
+            while True:
+                sp.tune_shape(shape, cap)
+                if cap >= cap_max:
+                    break
+                cap = min(cap * 2, cap_max)
+
     # -- admission -----------------------------------------------------------
     def _capacity(self, shape, waiting: int) -> int:
         npix = int(np.prod(shape))

@@ -239,6 +239,15 @@ _bind("spx_device_available", C.c_int)
 _bind("spx_last_error", C.c_char_p)
 _bind("spx_supported_length", C.c_int, C.c_int)
 _bind("spx_fft_variant", C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_bind("spx_tune_shape", C.c_int, C.c_int, C.c_int, C.c_int, C.c_int)
+
+
+def tune_shape(shape, nrec: int = 1) -> None:
+    """Runs the tuner for arrays of `shape` ((y, x) or (z, y, x)) in batches
+    of `nrec` now instead of at the first engine or batch of that shape."""
+    dims = (shape[-1], shape[-2], shape[0] if len(shape) == 3 else 1)
+    if spx_tune_shape(*dims, nrec) != 0:
+        raise ValueError(last_error())
 
 
 def fft_variant(L: int, tiled: bool = False) -> tuple[int, int]:

@@ -99,3 +99,22 @@ def test_admission_respects_free_memory(sp):
     srv2 = serve.Server(reserve_bytes=0, max_pixels_per_batch=1 << 20)
     assert srv2._capacity((64, 64), 5) == 8
     assert srv2._capacity((64, 64), 1000) == 256
+
+
+def test_prepare_tunes_ahead_without_changing_results(sp):
+    """spx_tune_shape / Server.prepare: tuning ahead for every batch size
+    leaves the served results equal to the same requests run alone; an
+    unsupported length is refused."""
+    from libspimage_amd import serve
+    shapes = [(40, 40), (36, 36, 36)]
+    reqs = _requests([((40, 40), "hio", 50), ((40, 40), "raar", 100), ((36, 36, 36), "er", 50)], seed0=700)
+    srv = serve.Server(quantum=50, max_pixels_per_batch=8 * 40 * 40, reserve_bytes=0, keep_models=True)
+    srv.prepare(shapes)
+    res = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    plain = serve.PlainRunner()
+    for q in reqs:
+        ef, fc, model = plain.run_one(q, keep_model=True)
+        assert np.array_equal(res[q.rid].model, model), q.rid
+    with pytest.raises(ValueError):
+        sp.tune_shape((7, 7), 2)
