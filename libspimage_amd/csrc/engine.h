@@ -54,6 +54,7 @@ struct Shape {
   LineGeom gy, gz;
   bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
   bool colw;     // 2-D row-major: the projection pass runs with the wider work-groups
+  int nt;        // 2-D: W loaded (bit 0) and stored (bit 1) with the non-temporal hint
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);

@@ -220,19 +220,25 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // reconstructions stay on one XCD, which favours the tiled layout, and more
 // of them fill the chip with wider work-groups).  Layouts and work-group
 // widths compute the same bits, so these choices never change a result.
-// Returns true for the tiled layout.  *wide: in the row-major layout, the projection pass runs
-// with the wider work-groups (LengthOps::col_project_wide), also timed here
-// with the chosen variants; SPX_COL_WIDE=0/1 forces the choice.  Caller
-// holds tune_mu.
+// Returns true for the tiled layout.  *wide: in the row-major layout, the
+// projection pass runs with the wider work-groups
+// (LengthOps::col_project_wide), also timed here with the chosen variants;
+// SPX_COL_WIDE=0/1 forces the choice.  *nt: the passes load and store W with
+// the non-temporal hint (3) or without (0).  W crosses XCDs between the
+// passes; one reconstruction gains from streaming it past L2 (1024^2: 20.1
+// against 27.3 us per iteration), batches lose (1024^2 x 16: 212 against
+// 201 us, 750^2 x 16: 209 against 167 us), so the final choice is timed
+// both ways; SPX_NT=0/1 forces it.  Caller holds tune_mu.
 struct PairChoice {
   bool t4, wide;
+  int nt;
 };
 static std::map<std::tuple<int, int, int>, PairChoice> tune_layout;
 
 static int batch_class(int nrec) { return nrec <= 1 ? 0 : nrec < 4 ? 1 : nrec < 8 ? 2 : 3; }
 
 static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
-                      bool* wide) {
+                      bool* wide, int* nt) {
   const auto dims = std::make_tuple(ex->L, ny, 1);
   auto known = shape_variants.find(dims);
   const bool need_row = known == shape_variants.end(), need_col = need_row;
@@ -248,7 +254,9 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     return t4 && possible;
   };
   const char* wenv = getenv("SPX_COL_WIDE");
+  const char* ntenv = getenv("SPX_NT");
   auto finish = [&](PairChoice pc) {
+    *nt = ntenv ? (atoi(ntenv) == 1 ? 3 : 0) : pc.nt;
     if (need_row) shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
     *vx = row_choice;
     *vy = col_choice;
@@ -260,8 +268,10 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   const auto key = std::make_tuple(ex->L, ny, batch_class(nrec));
   auto it = tune_layout.find(key);
   if (it != tune_layout.end()) return finish(it->second);
+  // the hint the layouts and variants are timed with
+  const int nt0 = batch_class(nrec) == 0 ? 3 : 0;
   if (tuning_off()) {
-    tune_layout[key] = PairChoice{false, false};
+    tune_layout[key] = PairChoice{false, false, nt0};
     return finish(tune_layout[key]);
   }
   const int nx = ex->L;
@@ -286,31 +296,35 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     ok = hipGetLastError() == hipSuccess;
   }
   float best = 1e30f;
-  int br = row_choice, bc = col_choice, blay = 0;
+  int br = row_choice, bc = col_choice, blay = 0, bnt = nt0;
   bool bwide = false;
   // lay 0: row-major, 1: tiled, 2: row-major with the wide projection pass
-  // (only for the best row-major variants, once lay 0 and 1 are timed)
+  // (only for the best row-major variants, once lay 0 and 1 are timed),
+  // 3: the best so far with the other non-temporal setting
   int r2 = -1, c2 = -1;
   float best_rm = 1e30f;
-  for (int lay = 0; ok && lay < 3; ++lay) {
-    const bool t4 = lay == 1, wd = lay == 2;
-    if (t4 && !t4_shape) continue;
-    if (wd && (r2 < 0 || !ey->var[c2].wide)) continue;
+  for (int lay = 0; ok && lay < 4; ++lay) {
+    const bool flip = lay == 3;
+    const bool t4 = flip ? blay == 1 : lay == 1, wd = flip ? bwide : lay == 2;
+    const int ntv = flip ? 3 - nt0 : nt0;
+    if (lay == 1 && !t4_shape) continue;
+    if (lay == 2 && (r2 < 0 || !ey->var[c2].wide)) continue;
     LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
     if (t4) {
       g.tshift = 2;
       g.tstride = 4L * ny;
     }
     for (int r = 0; r < ex->nvar; ++r) {
-      if ((!need_row && r != row_choice) || (t4 && !ex->var[r].t4) || (wd && r != r2)) continue;
+      if ((!need_row && r != row_choice) || (t4 && !ex->var[r].t4) || (lay == 2 && r != r2) || (flip && r != br))
+        continue;
       for (int c = 0; c < ey->nvar; ++c) {
-        if ((!need_col && c != col_choice) || (wd && c != c2)) continue;
+        if ((!need_col && c != col_choice) || (lay == 2 && c != c2) || (flip && c != bc)) continue;
         ColArgs ca{};
         ca.in = w;
         ca.W = w;
         ca.geo = g;
         ca.ampx = amp;
-        ca.nt = 1;
+        ca.nt = ntv;
         UpdateArgs u{};
         u.W = w;
         u.flags = fl;
@@ -320,7 +334,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
         u.beta = 0.9f;
         u.scale = 1.f;
         u.nsize = 1.f;
-        u.nt = 1;
+        u.nt = ntv;
         const Variant &vr = ex->var[r], &vc = ey->var[c];
         // steady state, iterates ping-ponging as in a run
         float2 *ga = g0, *gb = g1;
@@ -336,8 +350,9 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
             },
             20, 30);
         if (getenv("SPX_TUNE_LOG"))
-          fprintf(stderr, "tune %d x %d (x%d) %s: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
-                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), vr.E, vc.E, ms * 1000.f);
+          fprintf(stderr, "tune %d x %d (x%d) %s%s: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
+                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv ? ", non-temporal" : "", vr.E, vc.E,
+                  ms * 1000.f);
         if (lay == 0 && ms < best_rm) {
           best_rm = ms;
           r2 = r;
@@ -349,6 +364,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
           bc = c;
           blay = t4 ? 1 : 0;
           bwide = wd;
+          bnt = ntv;
         }
       }
     }
@@ -360,7 +376,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (s) (void)hipStreamDestroy(s);
-  tune_layout[key] = PairChoice{blay == 1, bwide};
+  tune_layout[key] = PairChoice{blay == 1, bwide, bnt};
   return finish(tune_layout[key]);
 }
 
@@ -737,7 +753,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt.tstride = 4L * ny;
   std::lock_guard<std::mutex> lock(tune_mu);
   if (found[0] && found[1] && !found[2]) {
-    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw);
+    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw, &sh->nt);
     return 0;
   }
   const auto key = std::make_tuple(nx, ny, nz);
@@ -1171,7 +1187,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   ca.centro = (p.constraints & kCentrosymmetricObject) ? 1 : 0;
   ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
   ca.objective = dm ? 0 : p.objective;
-  ca.nt = (nz == 1) ? 1 : 0;
+  ca.nt = (nz == 1) ? sh.nt : 0;
   ca.rec_active = some_idle ? rec_active : nullptr;
   ColArgs caf = ca;
   caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
@@ -1189,7 +1205,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.gamma2 = p.gamma2;
   u.scale = (float)(1.0 / (double)N);
   u.nsize = (float)N;
-  u.nt = (nz == 1) ? 1 : 0;
+  u.nt = (nz == 1) ? sh.nt : 0;
   u.rec_active = some_idle ? rec_active : nullptr;
   for (int i = 0; i < n; ++i) {
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));

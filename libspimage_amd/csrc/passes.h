@@ -42,12 +42,12 @@ enum : int {
 };
 
 // In 2-D iterations the row-transformed iterate W crosses XCDs between the
-// two passes (rows are spread over XCDs in one pass, columns in the other),
-// so those passes load and store it with the non-temporal hint (nt = 1): it
-// streams through L2 instead of displacing the per-XCD data that is reused
-// (iterates, flags, amplitudes) and leaves fewer dirty lines for the
-// end-of-kernel write-back.  3-D passes keep ordinary accesses (measured
-// faster there).
+// two passes (rows are spread over XCDs in one pass, columns in the other).
+// A single reconstruction's passes load (nt bit 0) and store (bit 1) it with
+// the non-temporal hint: it streams through L2 instead of displacing the
+// per-XCD data that is reused (iterates, flags, amplitudes) and leaves fewer
+// dirty lines for the end-of-kernel write-back.  Batches, and 3-D passes,
+// measure faster with ordinary accesses; the tuner decides per shape.
 typedef float spx_v2f __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st_nt(float2* p, float2 v) {
   const spx_v2f x = {v.x, v.y};
@@ -57,9 +57,9 @@ __device__ __forceinline__ float2 ld_nt(const float2* p) {
   const spx_v2f x = __builtin_nontemporal_load(reinterpret_cast<const spx_v2f*>(p));
   return make_float2(x.x, x.y);
 }
-__device__ __forceinline__ float2 ld_line(const float2* p, int nt) { return nt ? ld_nt(p) : *p; }
+__device__ __forceinline__ float2 ld_line(const float2* p, int nt) { return (nt & 1) ? ld_nt(p) : *p; }
 __device__ __forceinline__ void st_line(float2* p, float2 v, int nt) {
-  if (nt) st_nt(p, v);
+  if (nt & 2) st_nt(p, v);
   else *p = v;
 }
 
@@ -140,7 +140,7 @@ struct UpdateArgs {
   float gamma1, gamma2;
   float scale;             // 1/N as float
   float nsize;             // N as float (diff map divides by it)
-  int nt;                  // W / Wf with the non-temporal hint
+  int nt;                  // W / Wf loaded (bit 0) and stored (bit 1) with the non-temporal hint
 };
 
 enum ColMode : int {
@@ -170,7 +170,7 @@ struct ColArgs {
   // blur
   int nx, ny, nz;
   float sigma;
-  int nt;                    // line data with the non-temporal hint
+  int nt;                    // line data loaded (bit 0) and stored (bit 1) with the non-temporal hint
   // idle reconstructions (0 = skip the work-group), or null; outer index /
   // outer_per_rec is the reconstruction
   const uint8_t* rec_active;
