@@ -94,6 +94,10 @@ def test_init_error_codes_without_gpu(spm):
     assert spm.sp_phaser_iterate(ph, 1) == -3
     spm.sp_phaser_free(ph)
     assert spm.spx_fft_device(None, None, 8, 8, 1, 3) < 0
+    # tuning ahead needs the device, and refuses rather than touching it
+    assert spm.spx_tune_shape(64, 64, 1, 4) == -20
+    with pytest.raises(ValueError):
+        spm.tune_shape((64, 64), 4)
 
 
 def test_supported_lengths(spm):
