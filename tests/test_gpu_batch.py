@@ -176,3 +176,25 @@ def test_queued_efourier_equals_blocking(sp, shape):
     assert np.array_equal(b.model(), c.model())
     b.close()
     c.close()
+
+
+def test_pass_shapes_do_not_change_results(sp, monkeypatch):
+    """The tuner's free choices (wide or narrow projection pass, tiled or
+    row-major layout, non-temporal hint or not) compute the same bits: a
+    batch forced to the wide row-major pass without the hint equals single
+    phasers run with the defaults.  360 has 60 threads per line, so its
+    wide pass has 8 columns per work-group against 4."""
+    shape = (360, 360)
+    pats = patterns(shape, 3, seed=97)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    for k, v in (("SPX_COL_WIDE", "1"), ("SPX_NO_T4", "1"), ("SPX_NT", "0")):
+        monkeypatch.setenv(k, v)
+    b = make_batch(sp, pats, hio)
+    for k in ("SPX_COL_WIDE", "SPX_NO_T4", "SPX_NT"):
+        monkeypatch.delenv(k)
+    b.iterate(6)
+    got = b.model()
+    for r, p in enumerate(pats):
+        m, s, e = run_single(sp, p, "hio", 6)
+        assert np.array_equal(got[r], m), r
+    b.close()
