@@ -223,12 +223,17 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // Returns true for the tiled layout.  *wide: in the row-major layout, the
 // projection pass runs with the wider work-groups
 // (LengthOps::col_project_wide), also timed here with the chosen variants;
-// SPX_COL_WIDE=0/1 forces the choice.  *nt: the passes load and store W with
-// the non-temporal hint (3) or without (0).  W crosses XCDs between the
-// passes; one reconstruction gains from streaming it past L2 (1024^2: 20.1
-// against 27.3 us per iteration), batches lose (1024^2 x 16: 212 against
-// 201 us, 750^2 x 16: 209 against 167 us), so the final choice is timed
-// both ways; SPX_NT=0/1 forces it.  Caller holds tune_mu.
+// SPX_COL_WIDE=0/1 forces the choice.  *nt: which of the passes' streams
+// use the non-temporal hint (bit 0: W loads, 1: W stores, 2: g_k loads,
+// 3: g_{k+1} stores).  W crosses XCDs between the passes; one reconstruction
+// gains from streaming it past L2 (1024^2: 20.1 against 27.3 us per
+// iteration), batches lose (1024^2 x 16: 212 against 201 us, 750^2 x 16:
+// 209 against 167 us).  Streaming the iterates helps both, by different
+// masks (1024^2: 18.7 us with g_{k+1} stores only; 1024^2 x 16: 184 against
+// 201 us with both iterate streams).  The layout and variants are timed
+// with a default mask per batch class, then the chosen configuration with
+// every W setting (none or both) and iterate setting (none, either, both);
+// SPX_NT=<mask> forces it.  Caller holds tune_mu.
 struct PairChoice {
   bool t4, wide;
   int nt;
@@ -256,7 +261,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   const char* wenv = getenv("SPX_COL_WIDE");
   const char* ntenv = getenv("SPX_NT");
   auto finish = [&](PairChoice pc) {
-    *nt = ntenv ? (atoi(ntenv) == 1 ? 3 : 0) : pc.nt;
+    *nt = ntenv ? (atoi(ntenv) & 15) : pc.nt;
     if (need_row) shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
     *vx = row_choice;
     *vy = col_choice;
@@ -268,8 +273,8 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   const auto key = std::make_tuple(ex->L, ny, batch_class(nrec));
   auto it = tune_layout.find(key);
   if (it != tune_layout.end()) return finish(it->second);
-  // the hint the layouts and variants are timed with
-  const int nt0 = batch_class(nrec) == 0 ? 3 : 0;
+  // the hints the layouts and variants are timed with
+  const int nt0 = batch_class(nrec) == 0 ? 3 | 8 : 4 | 8;
   if (tuning_off()) {
     tune_layout[key] = PairChoice{false, false, nt0};
     return finish(tune_layout[key]);
@@ -300,13 +305,15 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   bool bwide = false;
   // lay 0: row-major, 1: tiled, 2: row-major with the wide projection pass
   // (only for the best row-major variants, once lay 0 and 1 are timed),
-  // 3: the best so far with the other non-temporal setting
+  // 3..10: the best so far with the eight hint masks other than nt0
+  static const int kMasks[8] = {0, 3, 4, 7, 8, 11, 12, 15};
   int r2 = -1, c2 = -1;
   float best_rm = 1e30f;
-  for (int lay = 0; ok && lay < 4; ++lay) {
-    const bool flip = lay == 3;
+  for (int lay = 0; ok && lay < 11; ++lay) {
+    const bool flip = lay >= 3;
     const bool t4 = flip ? blay == 1 : lay == 1, wd = flip ? bwide : lay == 2;
-    const int ntv = flip ? 3 - nt0 : nt0;
+    const int ntv = flip ? kMasks[lay - 3] : nt0;
+    if (flip && ntv == nt0) continue;
     if (lay == 1 && !t4_shape) continue;
     if (lay == 2 && (r2 < 0 || !ey->var[c2].wide)) continue;
     LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
@@ -350,9 +357,8 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
             },
             20, 30);
         if (getenv("SPX_TUNE_LOG"))
-          fprintf(stderr, "tune %d x %d (x%d) %s%s: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
-                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv ? ", non-temporal" : "", vr.E, vc.E,
-                  ms * 1000.f);
+          fprintf(stderr, "tune %d x %d (x%d) %s, hints %d: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
+                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv, vr.E, vc.E, ms * 1000.f);
         if (lay == 0 && ms < best_rm) {
           best_rm = ms;
           r2 = r;
@@ -377,6 +383,9 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     if (p) (void)hipFree(p);
   if (s) (void)hipStreamDestroy(s);
   tune_layout[key] = PairChoice{blay == 1, bwide, bnt};
+  if (getenv("SPX_TUNE_LOG"))
+    fprintf(stderr, "tune %d x %d (x%d) chose %s%s, hints %d: row E=%d col E=%d\n", nx, ny, nr,
+            blay == 1 ? "tiled" : "row-major", bwide ? " wide" : "", bnt, ex->var[br].E, ey->var[bc].E);
   return finish(tune_layout[key]);
 }
 

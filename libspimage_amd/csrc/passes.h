@@ -44,10 +44,12 @@ enum : int {
 // In 2-D iterations the row-transformed iterate W crosses XCDs between the
 // two passes (rows are spread over XCDs in one pass, columns in the other).
 // A single reconstruction's passes load (nt bit 0) and store (bit 1) it with
-// the non-temporal hint: it streams through L2 instead of displacing the
-// per-XCD data that is reused (iterates, flags, amplitudes) and leaves fewer
-// dirty lines for the end-of-kernel write-back.  Batches, and 3-D passes,
-// measure faster with ordinary accesses; the tuner decides per shape.
+// the non-temporal hint: it streams through L2 and leaves fewer dirty lines
+// for the end-of-kernel write-back.  The iterates g_k and g_{k+1} of the
+// real-space pass may stream too (bits 2 and 3).  Which streams gain depends
+// on the shape and the batch size (batches lose with W streamed: partial
+// lines of neighbouring column groups are then not merged); the tuner
+// decides per shape and batch class.  3-D passes use ordinary accesses.
 typedef float spx_v2f __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st_nt(float2* p, float2 v) {
   const spx_v2f x = {v.x, v.y};
@@ -140,7 +142,8 @@ struct UpdateArgs {
   float gamma1, gamma2;
   float scale;             // 1/N as float
   float nsize;             // N as float (diff map divides by it)
-  int nt;                  // W / Wf loaded (bit 0) and stored (bit 1) with the non-temporal hint
+  int nt;                  // non-temporal hint: W / Wf loads (bit 0) and stores (bit 1), g_k loads
+                           // (bit 2), g_{k+1} stores (bit 3)
 };
 
 enum ColMode : int {
@@ -425,7 +428,7 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     const long idx = base + out_pos<L, E>(t, q);
-    g0v[q] = a.gcur[idx];
+    g0v[q] = (a.nt & 4) ? ld_nt(&a.gcur[idx]) : a.gcur[idx];
     fl[q] = a.flags[idx];
   }
   TwRegs<L, E> twr;
@@ -473,8 +476,12 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
       g1.y -= beta * Pf.y / a.nsize;
       if (in) g1 = apply_constraints(g1, a.constraints);
     }
-    if (act) a.gnew[idx] = g1;
-    else if (idle_line) a.gnew[idx] = g0;
+    if (act) {
+      if (a.nt & 8) st_nt(&a.gnew[idx], g1);
+      else a.gnew[idx] = g1;
+    } else if (idle_line) {
+      a.gnew[idx] = g0;
+    }
     v[q] = g1;
     // f1 of the next iteration (src/phasing.c:1260-1274)
     if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
