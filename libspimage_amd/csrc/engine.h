@@ -54,7 +54,8 @@ struct Shape {
   LineGeom gy, gz;
   bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
   bool colw;     // 2-D row-major: the projection pass runs with the wider work-groups
-  int nt;        // 2-D: W loaded (bit 0) and stored (bit 1) with the non-temporal hint
+  int nt;        // non-temporal hints: W loads / stores (bits 0, 1; 2-D only), g_k loads /
+                 // g_{k+1} stores (bits 2, 3)
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);

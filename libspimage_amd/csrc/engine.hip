@@ -725,6 +725,8 @@ inline int rec_blocks(long n) {
 // ---------------------------------------------------------------------------
 // shapes and pass sequences shared by the engine and the one-shot transforms
 // ---------------------------------------------------------------------------
+static int tune_volume_hints(const Shape& sh0);
+
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   *sh = Shape{};
   sh->nx = nx;
@@ -769,12 +771,13 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   auto known = shape_variants.find(key);
   if (known != shape_variants.end()) {
     for (int a = 0; a < 3; ++a) ax[a]->v = known->second.v[a];
-    return 0;
+  } else {
+    if (found[0]) sh->ex.v = tune_length(found[0], ny * nz * nrec, nullptr);
+    if (found[1]) sh->ey.v = tune_length(found[1], 0, &sh->gy);
+    if (found[2]) sh->ez.v = tune_length(found[2], 0, &sh->gz);
+    shape_variants[key] = ShapeVariants{{sh->ex.v, sh->ey.v, sh->ez.v}};
   }
-  if (found[0]) sh->ex.v = tune_length(found[0], ny * nz * nrec, nullptr);
-  if (found[1]) sh->ey.v = tune_length(found[1], 0, &sh->gy);
-  if (found[2]) sh->ez.v = tune_length(found[2], 0, &sh->gz);
-  shape_variants[key] = ShapeVariants{{sh->ex.v, sh->ey.v, sh->ez.v}};
+  if (found[0] && found[1] && found[2]) sh->nt = tune_volume_hints(*sh);
   return 0;
 }
 
@@ -868,6 +871,85 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     if (mode == kColProject && !tiled && sh.colw) sh.ey->col_project_wide(a, s);
     else sh.ey->col(mode, a, s);
   }
+}
+
+// Non-temporal hints for the iterates of a 3-D shape's real-space pass
+// (UpdateArgs::nt bits 2 and 3; W keeps ordinary accesses in 3-D), timed
+// over whole iterations of the shape (x pass, y forward, z projection, y
+// inverse) on scratch, per shape and class of batch size.  Streaming both
+// iterates took 256^3 from 269 to 247 us per HIO iteration and 96^3 x 32
+// from 546 to 500 us, but 64^3 x 32 from 122 to 129 us.  SPX_NT=<mask>
+// forces the iterate bits.  Caller holds tune_mu.
+static std::map<std::tuple<int, int, int, int>, int> volume_hints;
+
+static int tune_volume_hints(const Shape& sh0) {
+  const char* ntenv = getenv("SPX_NT");
+  if (ntenv) return atoi(ntenv) & 12;
+  const auto key = std::make_tuple(sh0.nx, sh0.ny, sh0.nz, batch_class(sh0.nrec));
+  auto it = volume_hints.find(key);
+  if (it != volume_hints.end()) return it->second;
+  if (tuning_off()) return volume_hints[key] = 0;
+  const long N = (long)sh0.nx * sh0.ny * sh0.nz;
+  int nr = batch_class(sh0.nrec) == 3 && sh0.nrec < 16 ? 16 : sh0.nrec;
+  if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
+  if (nr < 1) nr = 1;
+  Shape sh = sh0;
+  sh.nrec = nr;
+  sh.gy = LineGeom{sh.nx, sh.nz * nr, (long)sh.nx * sh.ny, (long)sh.nx};
+  sh.gz = LineGeom{sh.nx * sh.ny, nr, N, (long)sh.nx * sh.ny};
+  const long n = (long)nr * N;
+  hipStream_t s = nullptr;
+  float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+  float* amp = nullptr;
+  uint8_t* fl = nullptr;
+  bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
+            hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+            hipMalloc(&fl, (size_t)n) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  int best_mask = 0;
+  float best = 1e30f;
+  for (int mask = 0; ok && mask <= 12; mask += 4) {
+    ColArgs ca{};
+    ca.ampx = amp;
+    UpdateArgs u{};
+    u.W = w;
+    u.flags = fl;
+    u.nlines = sh.ny * sh.nz * nr;
+    u.lines_per_rec = sh.ny * sh.nz;
+    u.alg = kHIO;
+    u.beta = 0.9f;
+    u.scale = 1.f;
+    u.nsize = 1.f;
+    u.nt = mask;
+    float2 *ga = g0, *gb = g1;
+    const float ms = time_launches(
+        s,
+        [&] {
+          enqueue_fourier(sh, kColProject, w, w, ca, s, false);
+          u.gcur = ga;
+          u.gnew = gb;
+          sh.ex->update(u, false, false, s);
+          std::swap(ga, gb);
+        },
+        2, 5);
+    if (getenv("SPX_TUNE_LOG"))
+      fprintf(stderr, "tune %d x %d x %d (x%d) iterate hints %d: %.2f us\n", sh.nx, sh.ny, sh.nz, nr, mask,
+              ms * 1000.f);
+    if (ms < best) {
+      best = ms;
+      best_mask = mask;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  void* bufs[] = {w, g0, g1, amp, fl};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (s) (void)hipStreamDestroy(s);
+  return volume_hints[key] = best_mask;
 }
 
 // column tiles per reconstruction of the last axis (the efourier partials)
@@ -1196,7 +1278,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   ca.centro = (p.constraints & kCentrosymmetricObject) ? 1 : 0;
   ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
   ca.objective = dm ? 0 : p.objective;
-  ca.nt = (nz == 1) ? sh.nt : 0;
+  ca.nt = sh.nt;  // 3-D shapes set only the iterate bits, which column passes ignore
   ca.rec_active = some_idle ? rec_active : nullptr;
   ColArgs caf = ca;
   caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
@@ -1214,7 +1296,7 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.gamma2 = p.gamma2;
   u.scale = (float)(1.0 / (double)N);
   u.nsize = (float)N;
-  u.nt = (nz == 1) ? sh.nt : 0;
+  u.nt = sh.nt;
   u.rec_active = some_idle ? rec_active : nullptr;
   for (int i = 0; i < n; ++i) {
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
