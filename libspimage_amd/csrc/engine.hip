@@ -594,18 +594,30 @@ __global__ void k_ctl_init(unsigned* ctl, int nrec, unsigned k) {
     c[kCtlK] = k;
     c[kCtlPrefix] = 0;
     c[kCtlThresh] = 0;
-    c[kCtlMax] = 0;
+    for (int x = 0; x < kHistSlices; ++x) c[kCtlMax + x] = 0;
   }
 }
 
 // one block per reconstruction
 __global__ __launch_bounds__(1024) void k_select(const unsigned* hist_all, int level, unsigned* ctl_all) {
   __shared__ unsigned scan[1024];
+  __shared__ unsigned lvl1[2048];
   const int nbins = (level == 3) ? 1024 : 2048;
-  const unsigned* hist = hist_all + (long)blockIdx.x * kHistStride + (level - 1) * 2048;
+  const unsigned* hist =
+      hist_all + (long)blockIdx.x * kHistStride + (level == 1 ? 0 : level == 2 ? kHistLevel2 : kHistLevel3);
   unsigned* ctl = ctl_all + blockIdx.x * kCtlStride;
   const int per = nbins / 1024;  // 1 or 2 bins per thread, walked high to low
   const int tid = threadIdx.x;
+  if (level == 1) {
+    // the per-XCD copies of the level-1 histogram, summed
+    for (int b = tid; b < 2048; b += 1024) {
+      unsigned c = 0;
+      for (int x = 0; x < kHistSlices; ++x) c += hist[x * 2048 + b];
+      lvl1[b] = c;
+    }
+    __syncthreads();
+    hist = lvl1;
+  }
   const unsigned k = ctl[kCtlK];
   unsigned local = 0;
   for (int q = 0; q < per; ++q) local += hist[nbins - 1 - (tid * per + q)];
@@ -638,7 +650,7 @@ __global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, u
                                                     int level) {
   __shared__ unsigned h[2048];
   const int nb = (level == 2) ? 2048 : 1024;
-  unsigned* hist = hist_all + (long)blockIdx.y * kHistStride + (level - 1) * 2048;
+  unsigned* hist = hist_all + (long)blockIdx.y * kHistStride + (level == 2 ? kHistLevel2 : kHistLevel3);
   for (int i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0u;
   __syncthreads();
   const unsigned pre = ctl_all[blockIdx.y * kCtlStride + kCtlPrefix];
@@ -660,7 +672,9 @@ __global__ __launch_bounds__(256) void k_hist_level(const float* absb, long n, u
 // (src/support_update.c:175).  Grid (blocks_per_rec, nrec).
 __global__ void k_mask(const float* absb, uint8_t* flags, long n, unsigned* ctl_all, int mode, float rel) {
   unsigned* ctl = ctl_all + blockIdx.y * kCtlStride;
-  const float t = (mode == 0) ? __uint_as_float(ctl[kCtlThresh]) : __uint_as_float(ctl[kCtlMax]) * rel;
+  unsigned mx = 0;
+  for (int x = 0; x < kHistSlices; ++x) mx = (ctl[kCtlMax + x] > mx) ? ctl[kCtlMax + x] : mx;
+  const float t = (mode == 0) ? __uint_as_float(ctl[kCtlThresh]) : __uint_as_float(mx) * rel;
   // record the threshold of mode 1 (mode 0 already holds it in kCtlThresh)
   if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlThresh] = __float_as_uint(t);
   REC_STRIDE(i, n) {
@@ -1498,7 +1512,7 @@ int Engine::blur_to_abs(float sigma, bool with_hist) {
   RowArgs r = row_args(sh, S1, nullptr);
   r.absb = absb;
   r.hist = with_hist ? hist : nullptr;
-  r.ctl = ctl;
+  r.ctl = with_hist ? nullptr : ctl;  // the maximum serves the threshold update only
   sh.ex->row(kRowBlurOut, r, s);
   SPX_HIP(hipGetLastError());
   return 0;

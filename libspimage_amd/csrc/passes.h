@@ -107,9 +107,17 @@ enum RowMode : int {
 
 // Per-reconstruction select scratch: 3 histograms of 2048 bins and a small
 // control block (k, prefix, threshold bits, max bits) for each reconstruction.
-constexpr int kHistStride = 3 * 2048;
-constexpr int kCtlStride = 8;
-enum CtlSlot { kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlMax = 3 };
+// Per-reconstruction select scratch.  The level-1 histogram (top 11 bits of
+// |blur|) is counted by a row pass with thousands of work-groups, whose
+// global atomics would all meet on the few bins the values fall in; it is
+// kept in kHistSlices copies, one per XCD (work-group x lands on XCD x % 8),
+// which the select sums.  Levels 2 and 3 follow.  The running maximum has
+// one slot per XCD in the same way.
+constexpr int kHistSlices = 8;
+constexpr int kHistLevel2 = kHistSlices * 2048, kHistLevel3 = kHistLevel2 + 2048;
+constexpr int kHistStride = kHistLevel3 + 2048;
+constexpr int kCtlStride = 16;
+enum CtlSlot { kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlMax = 8 /* .. 15, one per XCD */ };
 
 struct RowArgs {
   const float2* in;
@@ -268,6 +276,7 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
   __shared__ float2 shA[View::SIZE];
   __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
   __shared__ unsigned hsh[(MODE == kRowBlurOut) ? 2048 : 1];
+  __shared__ float wmax[(MODE == kRowBlurOut) ? (NT + 63) / 64 : 1];
   const int tid = threadIdx.x;
   const int g = tid / T, t = tid - g * T;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -302,6 +311,7 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
     const int rec_first = (blk * M) / a.lines_per_rec;
     const int last_line = (blk * M + M - 1) < a.nlines ? (blk * M + M - 1) : (a.nlines - 1);
     const bool one_rec = rec_first == last_line / a.lines_per_rec;
+    const int slice = blockIdx.x & (kHistSlices - 1);
     float mx = 0.f;
     if (a.hist) __syncthreads();  // histogram zeroing done
 #pragma unroll
@@ -312,22 +322,28 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_row(RowArgs a) {
         mx = fmaxf(mx, m);
         if (a.hist) {
           if (one_rec) atomicAdd(&hsh[__float_as_uint(m) >> 21], 1u);
-          else atomicAdd(&a.hist[(long)rec * kHistStride + (__float_as_uint(m) >> 21)], 1u);
+          else atomicAdd(&a.hist[(long)rec * kHistStride + slice * 2048 + (__float_as_uint(m) >> 21)], 1u);
         }
       }
     }
     if (a.ctl) {
+      // one atomic per work-group, into its XCD's slot
       if (one_rec) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-        if ((tid & 63) == 0) atomicMax(&a.ctl[rec_first * kCtlStride + kCtlMax], __float_as_uint(mx));
+        if ((tid & 63) == 0) wmax[tid >> 6] = mx;
+        __syncthreads();
+        if (tid == 0) {
+          for (int w = 1; w < (NT + 63) / 64; ++w) mx = fmaxf(mx, wmax[w]);
+          atomicMax(&a.ctl[rec_first * kCtlStride + kCtlMax + slice], __float_as_uint(mx));
+        }
       } else if (act) {
-        atomicMax(&a.ctl[rec * kCtlStride + kCtlMax], __float_as_uint(mx));
+        atomicMax(&a.ctl[rec * kCtlStride + kCtlMax + slice], __float_as_uint(mx));
       }
     }
     if (a.hist && one_rec) {
       __syncthreads();
-      unsigned* h = a.hist + (long)rec_first * kHistStride;
+      unsigned* h = a.hist + (long)rec_first * kHistStride + slice * 2048;
       for (int i = tid; i < 2048; i += NT)
         if (hsh[i]) atomicAdd(&h[i], hsh[i]);
     }
