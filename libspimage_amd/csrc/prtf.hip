@@ -86,21 +86,37 @@ __global__ __launch_bounds__(kThreads) void k_argmax_part(const float2* c, long 
 // The superposition decision (src/image_util.c:2983-3004): the shift is the
 // coordinates of the correlation peak, or of the convolution peak plus one
 // when that is strictly larger, which also reflects.  dec = {x, y, z, reflect}.
-__global__ void k_decide(const float* pv1, const int* pi1, const float* pv2, const int* pi2, int nparts,
-                         int enantio, int nx, int ny, int* dec) {
-  if (threadIdx.x != 0) return;
-  float v1 = -1.f;
-  int i1 = INT_MAX;
-  for (int k = 0; k < nparts; ++k) keep_best(v1, i1, pv1[k], pi1[k]);
-  int idx = i1, add = 0;
-  if (enantio) {
-    float v2 = -1.f;
-    int i2 = INT_MAX;
-    for (int k = 0; k < nparts; ++k) keep_best(v2, i2, pv2[k], pi2[k]);
-    if (v2 > v1) {
-      idx = i2;
-      add = 1;
+__global__ __launch_bounds__(kThreads) void k_decide(const float* pv1, const int* pi1, const float* pv2,
+                                                    const int* pi2, int nparts, int enantio, int nx, int ny,
+                                                    int* dec) {
+  // (value, index) pairs under keep_best form a total order, so the tree
+  // below finds the same first maximum as a scan in raster order
+  __shared__ float sv[2][kThreads];
+  __shared__ int si[2][kThreads];
+  const int tid = threadIdx.x;
+  float v1 = -1.f, v2 = -1.f;
+  int i1 = INT_MAX, i2 = INT_MAX;
+  for (int k = tid; k < nparts; k += kThreads) {
+    keep_best(v1, i1, pv1[k], pi1[k]);
+    if (enantio) keep_best(v2, i2, pv2[k], pi2[k]);
+  }
+  sv[0][tid] = v1;
+  si[0][tid] = i1;
+  sv[1][tid] = v2;
+  si[1][tid] = i2;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      keep_best(sv[0][tid], si[0][tid], sv[0][tid + off], si[0][tid + off]);
+      keep_best(sv[1][tid], si[1][tid], sv[1][tid + off], si[1][tid + off]);
     }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  int idx = si[0][0], add = 0;
+  if (enantio && sv[1][0] > sv[0][0]) {
+    idx = si[1][0];
+    add = 1;
   }
   dec[0] = idx % nx + add;
   dec[1] = (idx / nx) % ny + add;
@@ -165,18 +181,26 @@ __global__ __launch_bounds__(kThreads) void k_phase_part(const float2* a, const 
 
 // b = |b| exp(i (arg b + phi)), phi the angle of the summed partials
 // (src/image_util.c:3178-3184)
-__global__ void k_phase_apply(float2* b, long n, const double2* part, int nparts) {
-  __shared__ float phi_s;
-  if (threadIdx.x == 0) {
-    double sx = 0.0, sy = 0.0;
-    for (int k = 0; k < nparts; ++k) {
-      sx += part[k].x;
-      sy += part[k].y;
-    }
-    phi_s = (float)atan2(sy, sx);
+__global__ __launch_bounds__(kThreads) void k_phase_apply(float2* b, long n, const double2* part, int nparts) {
+  // every block sums the partials in the same fixed order
+  __shared__ double sx[kThreads], sy[kThreads];
+  const int tid = threadIdx.x;
+  double x = 0.0, y = 0.0;
+  for (int k = tid; k < nparts; k += kThreads) {
+    x += part[k].x;
+    y += part[k].y;
   }
+  sx[tid] = x;
+  sy[tid] = y;
   __syncthreads();
-  const float phi = phi_s;
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      sx[tid] += sx[tid + off];
+      sy[tid] += sy[tid + off];
+    }
+    __syncthreads();
+  }
+  const float phi = (float)atan2(sy[0], sx[0]);
   PRTF_STRIDE(i, n) {
     const float2 v = b[i];
     const float ang = atan2f(v.y, v.x) + phi;
@@ -307,7 +331,7 @@ int prtf_impl(const float* images, const int* masks, int nx, int ny, int nz, int
         enqueue_fft(sh, t2, t2, +1, s);
         hipLaunchKernelGGL(k_argmax_part, g, blk, 0, s, t2, N, pv + nparts, pi + nparts);
       }
-      hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, pv, pi, pv + nparts, pi + nparts, nparts,
+      hipLaunchKernelGGL(k_decide, dim3(1), dim3(kThreads), 0, s, pv, pi, pv + nparts, pi + nparts, nparts,
                          enantio ? 1 : 0, nx, ny, dec);
       hipLaunchKernelGGL(k_align, g, blk, 0, s, img, im, t1, mout, nx, ny, nz, dec);
       hipLaunchKernelGGL(k_phase_part, g, blk, 0, s, avg, t1, N, ph);
