@@ -56,6 +56,25 @@ def test_prtf_matches_oracle(shape, enantio):
     assert np.allclose(g["prtf_r"], o["prtf_r"], atol=1e-4)
 
 
+@pytest.mark.parametrize("n", [8, 9, 19])
+def test_prtf_over_chunk_boundaries(n):
+    """Reconstructions travel to and from the device in chunks of 8 beside
+    the alignment loop: counts on, just past and well past a chunk boundary
+    give the oracle's result."""
+    from libspimage_amd import prtf as gp
+    from oracle import prtf_oracle as P
+    imgs, masks = _copies((48, 80), n, seed=40 + n)
+    masks[n - 1] = 0  # a reconstruction without a mask in the last chunk
+    g = gp.prtf(imgs, masks, full_out=True)
+    o = P.prtf(imgs, masks, full_out=True)
+    assert np.array_equal(g["masks"], o["masks"])
+    for i in range(n):
+        assert rel(g["images"][i], o["images"][i]) < 1e-5, i
+    assert rel(g["super_image"], o["super_image"]) < 1e-5
+    assert np.abs(g["prtf"] - o["prtf"]).max() < 1e-4
+    assert np.array_equal(g["super_mask"], o["super_mask"])
+
+
 def test_prtf_without_translation():
     from libspimage_amd import prtf as gp
     from oracle import prtf_oracle as P
