@@ -66,6 +66,9 @@ int shape_variant_elements(int nx, int ny, int nz, int* e);
 // unnormalised transform of every reconstruction of `sh` (dir < 0 forward,
 // e^{-2 pi i k.n/N}); in == out is allowed
 void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s);
+// forward transform of |in| (as real complex values), the magnitude taken in
+// the x pass; needs an x axis longer than 1 (sh.ex)
+void enqueue_abs_fft(const Shape& sh, const float2* in, float2* out, hipStream_t s);
 
 class Engine {
  public:

@@ -814,10 +814,10 @@ static RowArgs row_args(const Shape& sh, const float2* in, float2* out) {
 }
 
 // FFT over all non-trivial axes: x by a row pass, then y and z by column passes
-void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
+static void enqueue_fft_impl(const Shape& sh, const float2* in, float2* out, int dir, bool abs_in, hipStream_t s) {
   const float2* src = in;
   if (sh.ex) {
-    sh.ex->row(dir < 0 ? kRowFwd : kRowInv, row_args(sh, in, out), s);
+    sh.ex->row(abs_in ? kRowAbsFwd : (dir < 0 ? kRowFwd : kRowInv), row_args(sh, in, out), s);
     src = out;
   }
   ColArgs c{};
@@ -838,6 +838,14 @@ void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStr
   if (src == in && in != out)
     (void)hipMemcpyAsync(out, in, sizeof(float2) * (size_t)sh.nx * sh.ny * sh.nz * sh.nrec, hipMemcpyDeviceToDevice,
                          s);
+}
+
+void enqueue_fft(const Shape& sh, const float2* in, float2* out, int dir, hipStream_t s) {
+  enqueue_fft_impl(sh, in, out, dir, false, s);
+}
+
+void enqueue_abs_fft(const Shape& sh, const float2* in, float2* out, hipStream_t s) {
+  enqueue_fft_impl(sh, in, out, -1, true, s);
 }
 
 // Column work after the x pass: the last non-trivial axis gets `mode`; the

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(kThreads) void k_phase_part(const float2* a, const 
 
 // b = |b| exp(i (arg b + phi)), phi the angle of the summed partials
 // (src/image_util.c:3178-3184)
-__global__ __launch_bounds__(kThreads) void k_phase_apply(float2* b, long n, const double2* part, int nparts) {
+__global__ __launch_bounds__(kThreads) void k_phase_apply(const float2* b, float2* out, long n, const double2* part,
+                                                          int nparts) {
   // every block sums the partials in the same fixed order
   __shared__ double sx[kThreads], sy[kThreads];
   const int tid = threadIdx.x;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void k_phase_apply(float2* b, long n, con
     const float2 v = b[i];
     const float ang = atan2f(v.y, v.x) + phi;
     const double m = (double)cabsf_ref(v);
-    b[i] = make_float2((float)(cos((double)ang) * m), (float)(sin((double)ang) * m));
+    out[i] = make_float2((float)(cos((double)ang) * m), (float)(sin((double)ang) * m));
   }
 }
 
@@ -320,10 +321,16 @@ int prtf_impl(const float* images, const int* masks, int nx, int ny, int nz, int
     const float2* cur = img;
     const int* curm = im;
     if (translate) {
-      hipLaunchKernelGGL(k_magnitude, g, blk, 0, s, avg, t1, N);
-      enqueue_fft(sh, t1, t1, -1, s);
-      hipLaunchKernelGGL(k_magnitude, g, blk, 0, s, img, t2, N);
-      enqueue_fft(sh, t2, t2, -1, s);
+      if (sh.ex) {
+        // the magnitudes taken in the transforms' x pass
+        enqueue_abs_fft(sh, avg, t1, s);
+        enqueue_abs_fft(sh, img, t2, s);
+      } else {
+        hipLaunchKernelGGL(k_magnitude, g, blk, 0, s, avg, t1, N);
+        enqueue_fft(sh, t1, t1, -1, s);
+        hipLaunchKernelGGL(k_magnitude, g, blk, 0, s, img, t2, N);
+        enqueue_fft(sh, t2, t2, -1, s);
+      }
       hipLaunchKernelGGL(k_corr_conv, g, blk, 0, s, t1, t2, t3, N, enantio ? 1 : 0);
       enqueue_fft(sh, t3, t3, +1, s);
       hipLaunchKernelGGL(k_argmax_part, g, blk, 0, s, t3, N, pv, pi);
@@ -335,9 +342,8 @@ int prtf_impl(const float* images, const int* masks, int nx, int ny, int nz, int
                          enantio ? 1 : 0, nx, ny, dec);
       hipLaunchKernelGGL(k_align, g, blk, 0, s, img, im, t1, mout, nx, ny, nz, dec);
       hipLaunchKernelGGL(k_phase_part, g, blk, 0, s, avg, t1, N, ph);
-      hipLaunchKernelGGL(k_phase_apply, g, blk, 0, s, t1, N, ph, nparts);
-      // the aligned reconstruction replaces the input in place
-      SPX_HIP(hipMemcpyAsync(img, t1, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+      // the aligned, phase-matched reconstruction replaces the input in place
+      hipLaunchKernelGGL(k_phase_apply, g, blk, 0, s, t1, img, N, ph, nparts);
       if (im) SPX_HIP(hipMemcpyAsync(im, mout, sizeof(int) * N, hipMemcpyDeviceToDevice, s));
       cur = img;
       curm = im;
