@@ -132,7 +132,9 @@ __device__ __forceinline__ int pmod(int v, int m) { return ((v % m) + m) % m; }
 // image but not the mask).  In 3-D the reflection leaves the middle plane
 // of an odd number of planes in place, as the reference's in-place swap loop
 // does (src/image_util.c:290-306).
-__global__ void k_align(const float2* img, const int* mask, float2* out, int* mout, int nx, int ny, int nz,
+// also counts, where cnt is given, the pixels outside the aligned mask
+// (the k_count_outside step of a reconstruction with a mask)
+__global__ void k_align(const float2* img, const int* mask, float2* out, int* mout, float* cnt, int nx, int ny, int nz,
                         const int* dec) {
   const int dx = dec[0], dy = dec[1], dz = dec[2], refl = dec[3];
   const long n = (long)nx * ny * nz;
@@ -142,7 +144,9 @@ __global__ void k_align(const float2* img, const int* mask, float2* out, int* mo
     const int y = (int)(r % ny), z = (int)(r / ny);
     const int px = pmod(x - dx, nx), py = pmod(y - dy, ny), pz = pmod(z - dz, nz);
     const long p = ((long)pz * ny + py) * nx + px;
-    if (mout) mout[q] = mask ? mask[p] : 0;
+    const int mv = mask ? mask[p] : 0;
+    if (mout) mout[q] = mv;
+    if (cnt && mv == 0) cnt[q] += 1.f;
     long s = p;
     if (refl && !(nz > 1 && (nz & 1) && pz == nz / 2)) s = ((long)(nz - 1 - pz) * ny + (ny - 1 - py)) * nx + (nx - 1 - px);
     out[q] = img[s];
@@ -181,8 +185,9 @@ __global__ __launch_bounds__(kThreads) void k_phase_part(const float2* a, const 
 
 // b = |b| exp(i (arg b + phi)), phi the angle of the summed partials
 // (src/image_util.c:3178-3184)
-__global__ __launch_bounds__(kThreads) void k_phase_apply(const float2* b, float2* out, long n, const double2* part,
-                                                          int nparts) {
+// also adds the result to the running sum avg (the k_add_c64 step)
+__global__ __launch_bounds__(kThreads) void k_phase_apply(const float2* b, float2* out, float2* avg, long n,
+                                                          const double2* part, int nparts) {
   // every block sums the partials in the same fixed order
   __shared__ double sx[kThreads], sy[kThreads];
   const int tid = threadIdx.x;
@@ -206,7 +211,9 @@ __global__ __launch_bounds__(kThreads) void k_phase_apply(const float2* b, float
     const float2 v = b[i];
     const float ang = atan2f(v.y, v.x) + phi;
     const double m = (double)cabsf_ref(v);
-    out[i] = make_float2((float)(cos((double)ang) * m), (float)(sin((double)ang) * m));
+    const float2 r = make_float2((float)(cos((double)ang) * m), (float)(sin((double)ang) * m));
+    out[i] = r;
+    avg[i] = make_float2(avg[i].x + r.x, avg[i].y + r.y);
   }
 }
 
@@ -340,16 +347,17 @@ int prtf_impl(const float* images, const int* masks, int nx, int ny, int nz, int
       }
       hipLaunchKernelGGL(k_decide, dim3(1), dim3(kThreads), 0, s, pv, pi, pv + nparts, pi + nparts, nparts,
                          enantio ? 1 : 0, nx, ny, dec);
-      hipLaunchKernelGGL(k_align, g, blk, 0, s, img, im, t1, mout, nx, ny, nz, dec);
+      hipLaunchKernelGGL(k_align, g, blk, 0, s, img, im, t1, mout, (im && has_mask[i]) ? cnt : nullptr, nx, ny, nz,
+                         dec);
       hipLaunchKernelGGL(k_phase_part, g, blk, 0, s, avg, t1, N, ph);
-      // the aligned, phase-matched reconstruction replaces the input in place
-      hipLaunchKernelGGL(k_phase_apply, g, blk, 0, s, t1, img, N, ph, nparts);
+      // the aligned, phase-matched reconstruction replaces the input in
+      // place and joins the running sum
+      hipLaunchKernelGGL(k_phase_apply, g, blk, 0, s, t1, img, avg, N, ph, nparts);
       if (im) SPX_HIP(hipMemcpyAsync(im, mout, sizeof(int) * N, hipMemcpyDeviceToDevice, s));
-      cur = img;
-      curm = im;
+    } else {
+      hipLaunchKernelGGL(k_add_c64, g, blk, 0, s, avg, cur, N);
+      if (curm && has_mask[i]) hipLaunchKernelGGL(k_count_outside, g, blk, 0, s, curm, cnt, N);
     }
-    hipLaunchKernelGGL(k_add_c64, g, blk, 0, s, avg, cur, N);
-    if (curm && has_mask[i]) hipLaunchKernelGGL(k_count_outside, g, blk, 0, s, curm, cnt, N);
     enqueue_fft(sh, cur, t2, -1, s);
     hipLaunchKernelGGL(k_unit_accum, g, blk, 0, s, t2, sum, zeros, N, 0);
   }
