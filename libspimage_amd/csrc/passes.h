@@ -263,7 +263,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV
 template <int L, int E>
 struct RowShape {
   static constexpr int T = PlanOf<L, E>::T;
-  static constexpr int M = (T >= 256) ? 1 : (256 / T);
+  // about 128 threads per work-group: short lines give more, smaller
+  // work-groups (256^2 alone 9.6 -> 9.1 us, 720^2 x 16 113 -> 109 us per
+  // iteration against 256 threads)
+  static constexpr int M = (T >= 128) ? 1 : (128 / T);
   static constexpr int NT = T * M;
 };
 
