@@ -241,12 +241,12 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
 
 
-def mixed_radix_bench(sizes=(720, 750, 810, 900, 972, 1000, 768, 960), nrec=16, iters=100, seed=7):
+def mixed_radix_bench(sizes=(720, 750, 810, 900, 972, 1000, 768, 960, 1024), nrec=16, iters=100, seed=7):
     """HIO iterations (device time, HIP events) of 2-D shapes whose lengths
-    are not powers of two, as the served mix has them: one reconstruction and
-    batches of `nrec`, per reconstruction-iteration and against the 78*N-byte
-    roofline.  The tuner picks layout, variants and the wide projection pass
-    per shape."""
+    are not powers of two, as the served mix has them, and of 1024^2 for
+    comparison: one reconstruction and batches of `nrec`, per
+    reconstruction-iteration and against the 78*N-byte roofline.  The tuner
+    picks layout, variants, column width and cache hints per shape."""
     from libspimage_amd import spimage as sp
     from libspimage_amd import workloads
     hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
