@@ -263,10 +263,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* NV
 template <int L, int E>
 struct RowShape {
   static constexpr int T = PlanOf<L, E>::T;
-  // about 128 threads per work-group: short lines give more, smaller
-  // work-groups (256^2 alone 9.6 -> 9.1 us, 720^2 x 16 113 -> 109 us per
-  // iteration against 256 threads)
-  static constexpr int M = (T >= 128) ? 1 : (128 / T);
+  // about 64 threads per work-group, 128 for lines of up to 128 points
+  // (3-D volumes up to 128^3): short lines give more, smaller work-groups.
+  // Against about 256 threads: 256^2 alone 9.6 -> 8.7 us, 720^2 alone 21.7
+  // -> 20.6 us, 720^2 x 16 113 -> 107 us per iteration; 96^3 x 32 prefers
+  // 120 threads to 60 (475 against 488 us).
+  static constexpr int M = (T >= 64) ? 1 : (L > 128 ? 64 / T : 128 / T);
   static constexpr int NT = T * M;
 };
 
