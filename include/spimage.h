@@ -7,4 +7,5 @@
 #include "sp_phasing.h"
 #include "sp_fft.h"
 #include "sp_hip_engine.h"
+#include "sp_cuda_util.h"
 #endif

@@ -1187,6 +1187,28 @@ void spx_host_free(void* p) {
 
 int spx_supported_length(int L) { return spx::find_length(L) != nullptr ? 1 : 0; }
 
+// ---------------------------------------------------------------------------
+// device queries of the reference's CUDA glue (include/sp_cuda_util.h;
+// reference src/cuda_util.c:10-29,39-60)
+// ---------------------------------------------------------------------------
+SpCUDADeviceType sp_cuda_get_device_type(void) {
+  return spx::device_available() ? SpCUDAHardwareDevice : SpCUDANoDevice;
+}
+
+void sp_cuda_launch_parameters(int64_t size, int* gridSize, int* blockSize) {
+  // HIP's limits: 1024 threads per block, 2^31 - 1 blocks along x
+  const int64_t max_block = 1024, max_grid = 2147483647;
+  int64_t block = 64;
+  while (size > max_grid * block) block *= 2;
+  if (block > max_block) {
+    fprintf(stderr, "sp_cuda_launch_parameters: %lld elements exceed %lld blocks of %lld threads\n",
+            (long long)size, (long long)max_grid, (long long)max_block);
+    abort();
+  }
+  *blockSize = (int)block;
+  *gridSize = (int)((size + block - 1) / block);
+}
+
 int spx_tune_shape(int nx, int ny, int nz, int nrec) {
   if (!spx::device_available()) {
     spx::set_error("no HIP device");

@@ -19,7 +19,7 @@ struct SpxBatch {
   std::vector<float> amp;     // host copy of the raw amplitudes (nrec * N)
   SpPhasingAlgorithm* alg = nullptr;
   SpSupportArray* sup = nullptr;
-  bool rec_tables = false;
+  bool rec_betas = false;  // per-reconstruction betas replace the schedule
   bool graphs = false;
   int iteration = 0;
 };
@@ -69,7 +69,9 @@ int run_phasing(SpxBatch* b, int n) {
     if (betas[i] != betas[0]) constant = false;
   }
   int rc;
-  if (b->graphs && n >= 3 && (constant || b->rec_tables)) {
+  // a graph replays one beta; per-reconstruction betas are read from the
+  // device table at run time, so either makes every replay exact
+  if (b->graphs && n >= 3 && (constant || b->rec_betas)) {
     // graph replays for all but the last iteration, which also writes gp
     rc = b->e->iterate_graph(p, betas[0], n - 1);
     if (!rc) rc = b->e->iterate(p, &betas[n - 1], 1, true);
@@ -198,7 +200,7 @@ int spx_batch_set_rec_algorithms(SpxBatch* b, const int* alg_types, const float*
     for (int r = 0; r < b->nrec; ++r)
       if (alg_types[r] != SpHIO && alg_types[r] != SpRAAR && alg_types[r] != SpER)
         return fail(-7, "per-reconstruction algorithms are ER, HIO or RAAR");
-  b->rec_tables = alg_types || betas;
+  b->rec_betas = betas != nullptr;
   return b->e->set_rec_params(alg_types, betas);
 }
 

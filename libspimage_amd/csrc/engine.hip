@@ -1133,16 +1133,25 @@ int Engine::download(const float2* dev, float2* host, int rec) {
 }
 
 // Captured graphs read the tables at run time, so new contents need no
-// recapture; only a table's first allocation changes the kernel arguments.
+// recapture; a table's first allocation or its removal changes the kernel
+// arguments.  A null table is freed, so iterations go back to the shared
+// algorithm type and beta schedule.
 int Engine::set_rec_params(const int* alg, const float* beta) {
-  if ((alg && !alg_rec) || (beta && !beta_rec)) drop_graphs();
+  if ((alg && !alg_rec) || (beta && !beta_rec) || (!alg && alg_rec) || (!beta && beta_rec)) drop_graphs();
+  SPX_HIP(hipStreamSynchronize(s));  // no queued iteration may still read a table freed below
   if (alg) {
     if (!alg_rec) SPX_HIP(hipMalloc(&alg_rec, sizeof(int) * nrec));
     SPX_HIP(hipMemcpyAsync(alg_rec, alg, sizeof(int) * nrec, hipMemcpyHostToDevice, s));
+  } else if (alg_rec) {
+    SPX_HIP(hipFree(alg_rec));
+    alg_rec = nullptr;
   }
   if (beta) {
     if (!beta_rec) SPX_HIP(hipMalloc(&beta_rec, sizeof(float) * nrec));
     SPX_HIP(hipMemcpyAsync(beta_rec, beta, sizeof(float) * nrec, hipMemcpyHostToDevice, s));
+  } else if (beta_rec) {
+    SPX_HIP(hipFree(beta_rec));
+    beta_rec = nullptr;
   }
   SPX_HIP(hipStreamSynchronize(s));
   return 0;

@@ -42,6 +42,7 @@ SpPixelInsideSupport, SpPixelMeasuredAmplitude = 1, 2
 SpRecoverPhases, SpRecoverAmplitudes = 0, 1
 SpSupportThreshold, SpSupportArea, SpSupportCentreImage, SpSupportTemplate, SpSupportStatic, SpSupportClose = 1, 2, 3, 4, 5, 6
 SP_COPY_DATA, SP_COPY_DETECTOR, SP_COPY_MASK, SP_COPY_ALL = 1, 0, 2, 3
+SpCUDANoDevice, SpCUDAEmulatedDevice, SpCUDAHardwareDevice = 0, 1, 2  # reference cuda_util.h:101
 
 
 # ---------------------------------------------------------------------------
@@ -233,6 +234,10 @@ _bind("sp_image_ifft", PImage, PImage)
 _bind("sp_image_fft_fast", None, PImage, PImage)
 _bind("sp_image_ifft_fast", None, PImage, PImage)
 _bind("sp_gaussian_blur", PImage, PImage, C.c_float)
+
+# device queries of the reference's CUDA glue (reference include/spimage/cuda_util.h:101-109)
+_bind("sp_cuda_get_device_type", C.c_int)
+_bind("sp_cuda_launch_parameters", None, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int))
 
 # plain-pointer engine ABI
 _bind("spx_device_available", C.c_int)

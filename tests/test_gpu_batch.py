@@ -198,3 +198,68 @@ def test_pass_shapes_do_not_change_results(sp, monkeypatch):
         m, s, e = run_single(sp, p, "hio", 6)
         assert np.array_equal(got[r], m), r
     b.close()
+
+
+def test_null_tables_restore_shared_algorithm(sp):
+    """spx_batch_set_rec_algorithms(b, NULL, NULL) drops the per-reconstruction
+    tables: the batch then follows the shared algorithm and beta schedule,
+    exactly like a batch that never had tables."""
+    pats = patterns((64, 64), 3, seed=61)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.set_rec_algorithms(np.array([4, 2, 2], np.int32), np.array([0.0, 0.5, 0.6], np.float32))
+    b.iterate(3)
+    # restart every slot from its start and drop the tables
+    for r, p in enumerate(pats):
+        b.set_model(r, p.start)
+    b.set_rec_algorithms(None, None)
+    b.iterate(6)
+    c = make_batch(sp, pats, hio)
+    c.iterate(6)
+    assert np.array_equal(b.model(), c.model())
+    b.close()
+    c.close()
+
+
+def test_graphs_with_algorithm_table_follow_beta_schedule(sp):
+    """Graphs on, per-reconstruction algorithm types but no per-reconstruction
+    betas, and a beta schedule that changes every iteration: each chunk equals
+    eager iteration (a replayed graph would freeze the first beta)."""
+    pats = patterns((64, 64), 3, seed=62)
+    ramp = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.95), (40, 0.6)]), 0)
+    algs = np.array([1, 2, 1], np.int32)
+    b_eager = make_batch(sp, pats, ramp, graphs=False)
+    b_graph = make_batch(sp, pats, ramp, graphs=True)
+    for b in (b_eager, b_graph):
+        b.set_rec_algorithms(algs, None)
+    for chunk in (6, 9, 5):
+        b_eager.iterate(chunk)
+        b_graph.iterate(chunk)
+        b_graph.synchronize()
+        assert np.array_equal(b_graph.model(), b_eager.model()), chunk
+    b_eager.close()
+    b_graph.close()
+
+
+@pytest.mark.parametrize("shape", [(256, 256), (384, 384), (512, 512)])
+def test_batch_sizes_of_the_batched_workload(sp, o32, shape):
+    """The batched workload's other size classes: every reconstruction of a
+    batch equals its single phaser bit for bit, and one HIO iteration of each
+    agrees with the FP32 oracle (relative L2 <= 1e-5)."""
+    from _helpers import rel
+    pats = patterns(shape, 2, seed=63)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.iterate(1)
+    one = b.model()
+    b.iterate(4)
+    got = b.model()
+    alg = o32.alg(1, 0, beta=[(0, 0.9)])
+    for r, p in enumerate(pats):
+        m, _, _ = run_single(sp, p, "hio", 5)
+        assert np.array_equal(got[r], m), r
+        oph = o32.phaser(shape)
+        oph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
+        assert oph.iterate(alg, 1) == 0
+        assert rel(one[r], oph.g1) <= 1e-5, r
+    b.close()
