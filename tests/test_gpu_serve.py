@@ -118,3 +118,20 @@ def test_prepare_tunes_ahead_without_changing_results(sp):
         assert np.array_equal(res[q.rid].model, model), q.rid
     with pytest.raises(ValueError):
         sp.tune_shape((7, 7), 2)
+
+
+def test_served_volumes_of_the_serving_mix(sp):
+    """The served workload's 3-D classes (64^3 and 128^3) mixed with a 2-D
+    class: each served result equals the request run alone, bit for bit."""
+    from libspimage_amd import serve
+    mix = [((64, 64, 64), "hio", 100), ((128, 128, 128), "er", 50), ((64, 64, 64), "raar", 50),
+           ((256, 256), "hio", 100), ((128, 128, 128), "hio", 100), ((64, 64, 64), "er", 150)]
+    reqs = _requests(mix, seed0=800)
+    srv = serve.Server(quantum=50, reserve_bytes=0, keep_models=True)
+    res = {r.rid: r for r in srv.run(reqs, realtime=True)}
+    srv.close()
+    plain = serve.PlainRunner()
+    for q in reqs:
+        ef, fc, model = plain.run_one(q, keep_model=True)
+        assert np.array_equal(res[q.rid].model, model), (q.rid, q.amplitudes.shape)
+        assert abs(res[q.rid].efourier - ef) <= 1e-6 * abs(ef), q.rid
