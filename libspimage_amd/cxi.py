@@ -1,8 +1,9 @@
 """CXI (HDF5) reading and writing for patterns in and reconstructions out.
 
-Mirrors the reference's ``spimage.CXIWriter`` and ``spimage.CXIReader``
-(src/_spimage_cxi.py:5-292) with the same constructors, methods and event
-dictionaries.  The reference builds on h5py, which this image does not
+Provides the interface of the reference's ``spimage.CXIWriter`` and
+``spimage.CXIReader`` (src/_spimage_cxi.py:5-292): the same constructors,
+methods, keyword arguments and event dictionaries.  Configuration errors raise
+ValueError instead of ending the process.  The reference builds on h5py, which this image does not
 have; this module binds the HDF5 C library (libhdf5, 1.10 API) directly
 with ctypes.
 
@@ -298,222 +299,224 @@ class H5File:
         self._io(path, index, data)
 
 
+def _leaves(tree, prefix=""):
+    """(path, value) for every non-dict value of a nested dictionary, and the
+    group paths on the way, depth first in the dictionary's order."""
+    for key, value in tree.items():
+        path = f"{prefix}/{key}"
+        if isinstance(value, dict):
+            yield path, None
+            yield from _leaves(value, path)
+        else:
+            yield path, value
+
+
+def _dataset_spec(value, n_events, per_event):
+    """Shape, dtype and CXI "axes" label of the dataset that holds `value`:
+    one entry per event (leading axis n_events) when `per_event`, else the
+    value alone (a scalar becomes a one-element vector)."""
+    arr = np.asarray(value)
+    if arr.dtype.kind in "SUO":
+        raise TypeError("CXI: string and object data are not supported")
+    if arr.ndim == 0:
+        return ((n_events,) if per_event else (1,)), arr.dtype, "experiment_identifier:value"
+    shape = ((n_events,) if per_event else ()) + arr.shape
+    return shape, arr.dtype, "experiment_identifier:y:x"
+
+
 class CXIWriter:
-    """reference src/_spimage_cxi.py:5-72: datasets of N events, written
-    whole or one event at a time; nested dictionaries become groups."""
+    """Writes a CXI file of `N` events (reference interface:
+    src/_spimage_cxi.py:5-72).
+
+    Nested dictionaries map onto HDF5 groups.  write_stack takes arrays whose
+    first axis runs over the N events and writes each one in a single call;
+    write_slice writes one event's values (i given) or values that belong to
+    the whole run (i None); write_to_dataset is the single-dataset form of
+    both.  Datasets are created on first use with the CXI "axes" attribute."""
 
     def __init__(self, filename, N, logger=None):
         self.filename = os.path.expandvars(filename)
-        self.f = H5File(self.filename, "w")
-        self.N = N
+        self.N = int(N)
         self.logger = logger
+        self.f = H5File(self.filename, "w")
+
+    def _ensure(self, path, value, per_event):
+        if path in self.f:
+            return
+        shape, dtype, axes = _dataset_spec(value, self.N, per_event)
+        self.f.create_dataset(path, shape, dtype, axes=axes)
+        if self.logger is not None:
+            self.logger.debug("created dataset %s %s %s", path, shape, dtype)
 
     def write_stack(self, data, name=""):
-        for i in range(self.N):
-            for k, d in data.items():
-                name_k = name + "/" + k
-                if isinstance(d, dict):
-                    if name_k not in self.f:
-                        self.f.create_group(name_k)
-                        self.write_stack(d, name_k)
-                else:
-                    d_s = d[i, :] if len(np.shape(d)) > 1 else d[i]
-                    self.write_to_dataset(d_s, name_k, i=i)
+        for path, value in _leaves(data, name):
+            if value is None:
+                self.f.create_group(path)
+                continue
+            stack = np.asarray(value)
+            if stack.ndim == 0 or stack.shape[0] != self.N:
+                raise ValueError(f"CXIWriter.write_stack: {path} needs {self.N} events along its first axis, "
+                                 f"got shape {stack.shape}")
+            self._ensure(path, stack[0], per_event=True)
+            self.f.write(path, stack)
 
     def write_slice(self, data, name="", i=None):
-        for k, d in data.items():
-            name_k = name + "/" + k
-            if isinstance(d, dict):
-                if name_k not in self.f:
-                    self.f.create_group(name_k)
-                self.write_slice(d, name_k, i=i)
+        for path, value in _leaves(data, name):
+            if value is None:
+                self.f.create_group(path)
             else:
-                self.write_to_dataset(d, name_k, i=i)
+                self.write_to_dataset(value, path, i=i)
 
     def write_to_dataset(self, data, name, i=None):
         if self.logger is not None:
-            self.logger.debug("Write dataset %s of event %s." % (name, i))
-        scalar = np.isscalar(data)
-        if name not in self.f:
-            if scalar:
-                s = [1] if i is None else [self.N]
-                t = np.asarray(data).dtype
-                axes = "experiment_identifier:value"
-            else:
-                data = np.asarray(data)
-                s = list(data.shape)
-                if i is not None:
-                    s.insert(0, self.N)
-                t = data.dtype
-                axes = "experiment_identifier:y:x"
-            if t.kind in "SUO":
-                raise TypeError(f"CXIWriter: string data ({name}) is not supported")
-            self.f.create_dataset(name, tuple(s), t, axes=axes)
+            self.logger.debug("writing %s (event %s)", name, i)
+        self._ensure(name, data, per_event=i is not None)
+        arr = np.asarray(data)
         if i is None:
-            if scalar:
-                self.f.write(name, np.asarray(data).reshape(1), index=None)
-            else:
-                self.f.write(name, np.asarray(data))
+            self.f.write(name, arr.reshape(1) if arr.ndim == 0 else arr)
         else:
-            self.f.write(name, np.asarray(data), index=i)
+            self.f.write(name, arr, index=int(i))
 
     def close(self):
         self.f.close()
 
 
 class CXIReader:
-    """reference src/_spimage_cxi.py:74-292: events of one .cxi file or of
-    every .cxi file in a directory, with the picking ("in_sequence",
-    "random"), filter (vmin/vmax, indices) and sorting options."""
+    """Reads events from one .cxi file or from every .cxi file of a
+    directory (reference interface: src/_spimage_cxi.py:74-292).
+
+    The events to deliver are decided once, at construction, as index arrays
+    over all files: a global keep-mask (filters on per-event datasets, then
+    the first `nevents` kept events from `ifirst` on, or a random subset of
+    `nevents`), and within each file the delivery order (natural, or by
+    ascending `sorting_dataset`).  get_next() then walks that schedule and
+    reads the stacked datasets at the scheduled event.
+
+    Keyword arguments: nevents (0 = all), ifirst, def_stack_ds (dataset whose
+    first axis counts a file's events; default "/i"), pick_events
+    ("in_sequence" or "random"), event_filters ({name: {"dataset_name": ...,
+    "vmin": ..., "vmax": ...} or {..., "indices": value or list}}),
+    sorting_dataset, dsnames_single ({key: dataset} read whole with every
+    event), logger."""
 
     def __init__(self, location, dsnames_stack={}, **kwargs):
         self.logger = kwargs.get("logger", None)
-        nevents = kwargs.get("nevents", 0)
-        ifirst = kwargs.get("ifirst", 0)
-        self.def_stack_ds = kwargs.get("def_stack_ds", "/i")
-        pick_events = kwargs.get("pick_events", "in_sequence")
-        event_filters = kwargs.get("event_filters", {})
-        sorting_dsname = kwargs.get("sorting_dataset", None)
-        self.dsnames_single = kwargs.get("dsnames_single", None)
         self.dsnames_stack = dsnames_stack
+        self.dsnames_single = kwargs.get("dsnames_single", None)
+        self.def_stack_ds = kwargs.get("def_stack_ds", "/i")
+        nevents = int(kwargs.get("nevents", 0))
+        ifirst = int(kwargs.get("ifirst", 0))
         self.directories, self.filenames = self._resolve_location(location)
-        self.ifile = 0
-        self.ifile_opened = None
-        self.F = None
         self.Nfiles = len(self.filenames)
         self.Nevents_files = self.get_Nevents_files()
         self.Nevents_tot = int(sum(self.Nevents_files))
         if nevents < 0:
-            sys.exit("ERROR: Events to read smaller 0. Change your configuration.")
-        elif nevents + ifirst > self.Nevents_tot:
-            sys.exit("ERROR: Not enough events to read. Change your configuration.")
-        to_process = [np.ones(N, dtype=bool) for N in self.Nevents_files]
-        to_process = self._filter_events(to_process, event_filters)
-        to_process = self._pick_events(to_process, pick_events, ifirst, nevents)
-        self.is_event_to_process = to_process
-        self.ievent_file = -1
+            raise ValueError(f"CXIReader: nevents must be >= 0 (got {nevents})")
+        if nevents + ifirst > self.Nevents_tot:
+            raise ValueError(f"CXIReader: ifirst {ifirst} + nevents {nevents} exceeds the "
+                             f"{self.Nevents_tot} events found")
+        starts = np.concatenate([[0], np.cumsum(self.Nevents_files)]).astype(np.int64)
+        keep = self._filtered(kwargs.get("event_filters", {}), starts)
+        keep = self._picked(keep, kwargs.get("pick_events", "in_sequence"), ifirst, nevents)
+        self.is_event_to_process = [keep[starts[f]:starts[f + 1]] for f in range(self.Nfiles)]
+        self.order = self._orders(kwargs.get("sorting_dataset", None))
+        # the delivery schedule: (file, event in file) of every event to read
+        files, events = [], []
+        for f in range(self.Nfiles):
+            sel = self.order[f][self.is_event_to_process[f][self.order[f]]]
+            files.append(np.full(sel.size, f, np.int64))
+            events.append(sel.astype(np.int64))
+        self._sched_file = np.concatenate(files) if files else np.zeros(0, np.int64)
+        self._sched_event = np.concatenate(events) if events else np.zeros(0, np.int64)
+        self.Nevents_process = int(self._sched_file.size)
         self.ievent_process = -1
-        self.Nevents_process = int(sum(t.sum() for t in to_process))
-        self.order = self._set_order(to_process, sorting_dsname)
+        self.ifile_opened = None
+        self.F = None
 
-    def _path(self, i):
-        return os.path.join(self.directories[i], self.filenames[i])
+    # -- the schedule ------------------------------------------------------
+    def _path(self, f):
+        return os.path.join(self.directories[f], self.filenames[f])
 
-    def get_next(self):
-        if self._next():
-            return self._get(self.dsnames_stack, self.dsnames_single)
-        return None
+    def _resolve_location(self, location0):
+        location = os.path.expandvars(location0)
+        if os.path.isdir(location):
+            names = sorted(n for n in os.listdir(location) if n.endswith(".cxi"))
+            return [location] * len(names), names
+        return [os.path.dirname(location)], [os.path.basename(location)]
 
     def get_Nevents_files(self):
+        counts = []
+        for f in range(len(self.filenames)):
+            with H5File(self._path(f)) as h:
+                counts.append(h.shape(self.def_stack_ds)[0])
+        if self.logger is not None:
+            for name, n in zip(self.filenames, counts):
+                self.logger.info("%s: %d events", name, n)
+        return counts
+
+    def _filtered(self, event_filters, starts):
+        keep = np.ones(self.Nevents_tot, bool)
+        for f in range(self.Nfiles):
+            part = keep[starts[f]:starts[f + 1]]
+            with H5File(self._path(f)) as h:
+                for fname, flt in event_filters.items():
+                    values = np.ravel(h.read(flt["dataset_name"]))
+                    if "vmin" in flt and "vmax" in flt:
+                        ok = (values >= flt["vmin"]) & (values <= flt["vmax"])
+                    elif "indices" in flt:
+                        ok = np.isin(values, np.atleast_1d(np.asarray(flt["indices"])))
+                    else:
+                        if self.logger is not None:
+                            self.logger.warning("filter %s has neither vmin/vmax nor indices", fname)
+                        continue
+                    part &= ok
+        return keep
+
+    def _picked(self, keep, mode, ifirst, nevents):
+        if mode == "in_sequence":
+            keep[:ifirst] = False
+            if nevents:
+                kept = np.flatnonzero(keep)
+                keep[kept[nevents:]] = False
+        elif mode == "random":
+            kept = np.flatnonzero(keep)
+            chosen = np.random.default_rng().permutation(kept)[:nevents]
+            keep = np.zeros_like(keep)
+            keep[chosen] = True
+        else:
+            raise ValueError(f"CXIReader: unknown pick_events mode {mode!r}")
+        return keep
+
+    def _orders(self, sorting_dsname):
+        if sorting_dsname is None:
+            return [np.arange(n) for n in self.Nevents_files]
         out = []
-        for i in range(len(self.filenames)):
-            with H5File(self._path(i)) as f:
-                out.append(f.shape(self.def_stack_ds)[0])
+        for f in range(self.Nfiles):
+            with H5File(self._path(f)) as h:
+                out.append(np.argsort(np.ravel(h.read(sorting_dsname)), kind="stable"))
         return out
+
+    # -- delivery ----------------------------------------------------------
+    def get_next(self):
+        if self.ievent_process + 1 >= self.Nevents_process:
+            self.close()
+            return None
+        self.ievent_process += 1
+        f = int(self._sched_file[self.ievent_process])
+        e = int(self._sched_event[self.ievent_process])
+        if self.ifile_opened != f:
+            self.close()
+            self.F = H5File(self._path(f))
+            self.ifile_opened = f
+        event = {"i": self.ievent_process, "filename": self.filenames[f], "filename_i": f}
+        for key, dsname in (self.dsnames_stack or {}).items():
+            event[key] = self.F.read(dsname, index=e)
+        for key, dsname in (self.dsnames_single or {}).items():
+            event[key] = self.F.read(dsname)
+        return event
 
     def close(self):
         if self.F is not None:
             self.F.close()
             self.F = None
-
-    def _resolve_location(self, location0):
-        location = os.path.expandvars(location0)
-        if os.path.isdir(location):
-            fs = sorted(x for x in os.listdir(location) if x[-4:] == ".cxi")
-            return [location + "/"] * len(fs), fs
-        fname = location.split("/")[-1]
-        return [location[:-len(fname)]], [fname]
-
-    def _pick_events(self, to_process, mode, ifirst, nevents):
-        temp = np.concatenate(to_process) if to_process else np.zeros(0, bool)
-        if mode == "in_sequence":
-            temp[:ifirst] = False
-            if nevents != 0 and nevents < temp.sum():
-                s = 0
-                i = ifirst
-                for i in range(ifirst, self.Nevents_tot):
-                    if temp[i]:
-                        s += 1
-                    if s == nevents:
-                        break
-                temp[i + 1:] = False
-        elif mode == "random":
-            rng = np.random.default_rng()
-            pool = list(np.arange(self.Nevents_tot)[temp])
-            temp = np.zeros_like(temp)
-            for _ in range(nevents):
-                if len(pool) != 1:
-                    temp[pool.pop(int(rng.integers(len(pool))))] = True
-                else:
-                    temp[pool[0]] = True
-        else:
-            raise ValueError(f"no valid picking mode chosen: {mode}")
-        out, i = [], 0
-        for N in self.Nevents_files:
-            out.append(temp[i:i + N])
-            i += N
-        return out
-
-    def _filter_events(self, to_process, event_filters):
-        for i in range(self.Nfiles):
-            with H5File(self._path(i)) as f:
-                for flt in event_filters.values():
-                    ds = f.read(flt["dataset_name"]).flatten()
-                    if "vmin" in flt and "vmax" in flt:
-                        F = (ds >= flt["vmin"]) & (ds <= flt["vmax"])
-                    elif "indices" in flt:
-                        F = np.zeros_like(to_process[i])
-                        for index in (flt["indices"] if isinstance(flt["indices"], list) else [flt["indices"]]):
-                            F[ds == index] = True
-                    else:
-                        if self.logger is not None:
-                            self.logger.warning("No valid filter arguments given!")
-                        F = np.ones_like(to_process[i])
-                    to_process[i] = to_process[i] & F
-        return to_process
-
-    def _set_order(self, to_process, sorting_dsname):
-        order = []
-        for i, N in enumerate(self.Nevents_files):
-            if sorting_dsname is None:
-                order.append(np.arange(N))
-            else:
-                with H5File(self._path(i)) as f:
-                    order.append(np.argsort(f.read(sorting_dsname)))
-        return order
-
-    def _next(self):
-        while True:
-            if self.ievent_process == self.Nevents_process - 1:
-                return False
-            self.ievent_file += 1
-            if self.ifile >= self.Nfiles:
-                self.close()
-                return False
-            if self.ievent_file >= self.Nevents_files[self.ifile]:
-                self.ifile += 1
-                self.ievent_file = 0
-                if self.ifile >= self.Nfiles:
-                    self.close()
-                    return False
-            if self.is_event_to_process[self.ifile][self.order[self.ifile][self.ievent_file]]:
-                self.ievent_process += 1
-                return True
-
-    def _open_file(self):
-        if self.ifile_opened != self.ifile:
-            self.close()
-            self.F = H5File(self._path(self.ifile))
-            self.ifile_opened = self.ifile
-
-    def _get(self, dsnames_stack, dsnames_single):
-        self._open_file()
-        D = {"i": self.ievent_process, "filename": self.filenames[self.ifile_opened],
-             "filename_i": self.ifile_opened}
-        if dsnames_stack is not None:
-            for key, dsname in dsnames_stack.items():
-                D[key] = self.F.read(dsname, index=int(self.order[self.ifile][self.ievent_file]))
-        if dsnames_single is not None:
-            for key, dsname in dsnames_single.items():
-                D[key] = self.F.read(dsname)
-        return D
+            self.ifile_opened = None

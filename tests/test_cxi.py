@@ -146,3 +146,33 @@ def test_writer_rejects_strings(tmp_path):
     with pytest.raises(TypeError):
         w.write_slice({"name": np.array(["x"])})
     w.close()
+
+
+def test_reader_configuration_errors_raise(sample):
+    """Bad event counts raise ValueError; the library never ends the process."""
+    path, data, N = sample
+    with pytest.raises(ValueError):
+        cxi.CXIReader(str(path), {}, def_stack_ds="/entry_1/photons", nevents=-1)
+    with pytest.raises(ValueError):
+        cxi.CXIReader(str(path), {}, def_stack_ds="/entry_1/photons", ifirst=3, nevents=N)
+    with pytest.raises(ValueError):
+        cxi.CXIReader(str(path), {}, def_stack_ds="/entry_1/photons", pick_events="sideways")
+
+
+def test_reader_random_pick_is_a_subset(sample):
+    path, data, N = sample
+    r = cxi.CXIReader(str(path), {"p": "/entry_1/photons"}, def_stack_ds="/entry_1/photons",
+                      pick_events="random", nevents=3)
+    got = []
+    while (d := r.get_next()) is not None:
+        got.append(float(d["p"]))
+    assert len(got) == 3 and len(set(got)) == 3
+    assert set(got) <= set(data["entry_1"]["photons"].tolist())
+
+
+def test_writer_stack_needs_n_events(tmp_path):
+    _h5lib_or_skip()
+    w = cxi.CXIWriter(str(tmp_path / "n.cxi"), 4)
+    with pytest.raises(ValueError):
+        w.write_stack({"x": np.zeros((3, 2))})
+    w.close()
