@@ -40,6 +40,7 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_HIO = 78             # algorithmic bytes per pixel of one 2-D HIO iteration
 BYTES_REAL_PASS = 41       # inverse + forward x passes (32) + g0 (8) + support mask (1)
 BYTES_FOURIER_PASS = 37    # forward + inverse y passes (32) + amplitudes (4) + measured mask (1)
+SAMPLES = 5               # headline: median of this many timed samples
 MKL_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.2", "/usr/lib/x86_64-linux-gnu/libmkl_rt.so"]
 
 
@@ -55,6 +56,7 @@ def parse():
     ap.add_argument("--no-volume", action="store_true", help="skip the 256^3 reconstruction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-spec", default="{}", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only launch 20 plain 64 MiB device-to-device copies (counter calibration)")
     return ap.parse_args()
@@ -80,7 +82,56 @@ def make_hio_phaser(sp, pattern, beta=0.9, alg="hio", sup_array=None):
     return ph, (a, imgs, keep)
 
 
-def gpu_bench(args):
+def gpu_one_step(sp, kind, beta, amp, mask, support, model):
+    """One GPU iteration of `kind` from `model` with a fresh phaser (no
+    support updates): the state after it, for the accuracy probes."""
+    import math
+    bmap = sp.smap([(0, beta)])
+    if kind == "hio":
+        a = sp.sp_phasing_hio_alloc(bmap, 0)
+    elif kind == "raar":
+        a = sp.sp_phasing_raar_alloc(bmap, 0)
+    elif kind == "dm":
+        a = sp.sp_phasing_diff_map_alloc(bmap, math.inf, math.inf, 0)
+    else:
+        a = sp.sp_phasing_er_alloc(0)
+    ph = sp.sp_phaser_alloc()
+    imgs = [sp.image_from_array(amp, mask), sp.image_from_array(model), sp.image_from_array(support)]
+    try:
+        if sp.sp_phaser_init(ph, a, None, sp.SpEngineHIP) != 0:
+            raise RuntimeError(sp.last_error())
+        sp.sp_phaser_set_amplitudes(ph, imgs[0])
+        if sp.sp_phaser_init_model(ph, imgs[1], 0) != 0 or sp.sp_phaser_init_support(ph, imgs[2], 0, 0.0) != 0:
+            raise RuntimeError(sp.last_error())
+        if sp.sp_phaser_iterate(ph, 1) != 0:
+            raise RuntimeError(sp.last_error())
+        return sp.image_view(sp.sp_phaser_model(ph)).copy()
+    finally:
+        sp.sp_phaser_free(ph)
+        for im in imgs:
+            sp.sp_image_free(im)
+
+
+def phaser_state(sp, ph):
+    """(current model, current support) of a phaser, as host arrays."""
+    import numpy as np
+    model = sp.image_view(sp.sp_phaser_model(ph)).copy()
+    sup = (sp.image_view(sp.sp_phaser_support(ph)).real > 0).astype(np.float32)
+    return model, sup
+
+
+def save_probe(probe_dir, name, kind, beta, pattern, support, before, after):
+    """A state before one GPU iteration and the GPU's state after it; the CPU
+    leg repeats the iteration in the FP64 oracle."""
+    if not probe_dir:
+        return
+    import numpy as np
+    np.savez(Path(probe_dir) / f"{name}.npz", kind=kind, beta=beta, amp=pattern.amplitudes.astype(np.float32),
+             mask=pattern.mask.astype(np.uint8), support=np.asarray(support).astype(np.uint8),
+             before=before.astype(np.complex64), after=after.astype(np.complex64))
+
+
+def gpu_bench(args, probe_dir=None):
     from libspimage_amd import spimage as sp
     from libspimage_amd import workloads
     if not sp.spx_device_available():
@@ -89,11 +140,16 @@ def gpu_bench(args):
     N = p.amplitudes.size
     ph, keep = make_hio_phaser(sp, p)
     ms = C.c_float()
-    if sp.spx_phaser_time_iterations(ph, args.warmup, C.byref(ms)) != 0:
+    if sp.spx_phaser_time_iterations(ph, max(args.warmup, 1), C.byref(ms)) != 0:
         raise RuntimeError(sp.last_error())
-    if sp.spx_phaser_time_iterations(ph, args.steps, C.byref(ms)) != 0:
-        raise RuntimeError(sp.last_error())
-    t_iter = ms.value / 1e3 / args.steps
+    # the median of SAMPLES samples of at least 100 iterations each
+    n_sample = max(args.steps, 100)
+    samples = []
+    for _ in range(SAMPLES):
+        if sp.spx_phaser_time_iterations(ph, n_sample, C.byref(ms)) != 0:
+            raise RuntimeError(sp.last_error())
+        samples.append(ms.value / 1e3 / n_sample)
+    t_iter = sorted(samples)[SAMPLES // 2]
     # per-pass durations (events between the two passes of each iteration)
     mf, mr = C.c_float(), C.c_float()
     npass = max(200, args.steps // 4)
@@ -106,6 +162,9 @@ def gpu_bench(args):
     gap = max(0.0, (t_f_ev + t_r_ev - t_iter) / 2)
     t_f, t_r = t_f_ev - gap, t_r_ev - gap
     tiled = sp.spx_phaser_tiled(ph) == 1
+    before, sup_now = phaser_state(sp, ph)
+    save_probe(probe_dir, "hio_1024", "hio", 0.9, p, sup_now, before,
+               gpu_one_step(sp, "hio", 0.9, p.amplitudes, p.mask, sup_now, before))
     sp.sp_phaser_free(ph)
 
     passes = {
@@ -135,6 +194,8 @@ def gpu_bench(args):
         "passes_us": {k: v["seconds"] * 1e6 for k, v in passes.items()},
         "passes_event_bracketed_us": {k: v["event_seconds"] * 1e6 for k, v in passes.items()},
         "fft_elements_per_thread": dict(zip(("row", "column"), sp.fft_variant(1024, tiled))),
+        "timing": f"median of {SAMPLES} samples of {n_sample} iterations (HIP events on the engine stream)",
+        "samples_us_per_iteration": [t * 1e6 for t in samples],
         "fourier_space_layout": "4-column tiled" if tiled else "row-major",
     }
 
@@ -156,9 +217,47 @@ def gpu_bench(args):
     recon = {"workload": "RAAR 1024x1024, beta 0.87, 300 iterations, area shrinkwrap every 20",
              "rc": rc, "seconds": t_rec, "iter_per_s": 300 / t_rec, "final_efourier": ef,
              "support_fraction": sp.sp_phaser_support_fraction(ph3)}
+    before, sup_now = phaser_state(sp, ph3)
+    save_probe(probe_dir, "raar_1024_shrinkwrap", "raar", 0.87, p, sup_now, before,
+               gpu_one_step(sp, "raar", 0.87, p.amplitudes, p.mask, sup_now, before))
     sp.sp_phaser_free(ph2)
     sp.sp_phaser_free(ph3)
     return t_iter, roofline, iteration_roofline, recon
+
+
+def disc_bench(probe_dir=None):
+    """The 256^2 disc workload in full on the GPU: HIO beta 0.9 for 200
+    iterations, then ER for 20, static support, from the pattern's
+    random-phase start; wall time from the first iteration to a
+    synchronising error read-back."""
+    from libspimage_amd import spimage as sp
+    from libspimage_amd import workloads
+    p = workloads.disc_256(0)
+    er = sp.sp_phasing_er_alloc(0)
+    ph, keep = make_hio_phaser(sp, p)
+    sp.spx_phaser_time_iterations(ph, 20, C.byref(C.c_float()))  # warm-up (tuning, code objects)
+    sp.sp_phaser_free(ph)
+    ph, keep = make_hio_phaser(sp, p)
+    sp.sp_phaser_efourier(ph, None)
+    t0 = time.perf_counter()
+    rc = sp.sp_phaser_iterate(ph, 200)
+    rc = rc or sp.sp_phaser_set_phasing_algorithm(ph, er) or sp.sp_phaser_iterate(ph, 20)
+    fc = C.c_float()
+    ef = sp.sp_phaser_efourier(ph, C.byref(fc))
+    t = time.perf_counter() - t0
+    out = {"workload": "256x256 disc pattern (20 Gaussians, 1e5 photons, beamstop 4 px), static support: HIO beta 0.9 "
+                       "200 iterations then ER 20", "rc": rc, "seconds": t, "final_efourier": ef, "fcfo": fc.value,
+           "ereal": sp.sp_phaser_ereal(ph)}
+    sp.sp_phaser_free(ph)
+    if probe_dir:
+        # one HIO step from the state at iteration 200 of the same run
+        ph, keep = make_hio_phaser(sp, p)
+        sp.sp_phaser_iterate(ph, 200)
+        before, sup_now = phaser_state(sp, ph)
+        sp.sp_phaser_free(ph)
+        save_probe(probe_dir, "hio_256_disc", "hio", 0.9, p, sup_now, before,
+                   gpu_one_step(sp, "hio", 0.9, p.amplitudes, p.mask, sup_now, before))
+    return out
 
 
 def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=2, seed=2000):
@@ -270,7 +369,7 @@ def mixed_radix_bench(sizes=(720, 750, 810, 900, 972, 1000, 768, 960, 1024), nre
                         f"{iters} iterations timed", "shapes": out}
 
 
-def volume_bench(dm_its=200, er_its=100, period=20):
+def volume_bench(dm_its=200, er_its=100, period=20, probe_dir=None):
     """3-D reconstruction of the 256^3 ellipsoid pattern: difference map
     (beta 0.8, gamma1 = gamma2 = infinity -> Elser's values) for `dm_its`
     iterations, then ER for `er_its`, with an area shrinkwrap every `period`
@@ -312,6 +411,10 @@ def volume_bench(dm_its=200, er_its=100, period=20):
         t = ms.value / 1e3 / 40
         per_it[name] = {"us_per_iteration": t * 1e6, "algorithmic_bytes": nbytes * N,
                         "achieved_GBps": nbytes * N / t / 1e9, "frac": nbytes * N / t / PEAK_HBM}
+        if name == "dm" and probe_dir:
+            before, sup_now = phaser_state(sp, ph)
+            save_probe(probe_dir, "dm_256cubed", "dm", 0.8, p, sup_now, before,
+                       gpu_one_step(sp, "dm", 0.8, p.amplitudes, p.mask, sup_now, before))
         sp.sp_phaser_free(ph)
     ph = phaser(dm, sarr)
     sp.sp_phaser_efourier(ph, None)
@@ -482,16 +585,52 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
             "efourier_max_rel_diff_served_vs_plain": diff}
 
 
-def cpu_baseline_worker(seconds: float):
-    """Runs in a child process: the oracle's HIO loop on one core."""
-    from libspimage_amd import workloads
+# ---------------------------------------------------------------------------
+# CPU legs (a child process; the only place the oracle runs)
+# ---------------------------------------------------------------------------
+def _cpu_threads() -> int:
+    """Host threads for the all-cores CPU figure: this process's CPU share
+    (the GPU box exports it as OMP_NUM_THREADS) within the affinity mask."""
+    avail = len(os.sched_getaffinity(0))
+    for var in ("SPX_CPU_THREADS", "OMP_NUM_THREADS"):
+        v = os.environ.get(var, "")
+        if v.isdigit() and int(v) > 0:
+            return min(int(v), avail)
+    return avail
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_oracle(threads: int):
+    """The oracle's FP32 build timing the reference loop: MKL's FFTW3 API,
+    planned per call like src/fft.c:177-189, with `threads` plan threads
+    (the reference's sp_init_fft, src/fft.c:34-49; its phaser never calls
+    it, so 1 is the reference default)."""
     from oracle.oracle import Oracle
     o = Oracle("f32")
     backend = "own double-precision FFT"
     for path in MKL_CANDIDATES:
         if os.path.exists(path) and o.set_fft_backend(1, path) == 0:
-            backend = f"FFTW3 API of {os.path.basename(path)} (MKL, sequential), plan per call"
+            backend = f"FFTW3 API of {os.path.basename(path)} (MKL), plan per call"
+            if threads > 1:
+                backend += f", {threads} FFT threads" if o.set_fft_threads(threads) == 0 else \
+                    ", FFT threads unavailable"
             break
+    return o, backend
+
+
+def _cpu_hio_1024(o, seconds):
+    """The headline loop: HIO beta 0.9 on the 1024^2 blob pattern, static
+    support (src/phasing.c:1142-1172)."""
+    from libspimage_amd import workloads
     p = workloads.blob_1024()
     ph = o.phaser(p.amplitudes.shape)
     ph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
@@ -503,19 +642,168 @@ def cpu_baseline_worker(seconds: float):
         ph.iterate(alg, 1)
         n += 1
     dt = time.perf_counter() - t0
-    print(json.dumps({"iters": n, "seconds": dt, "backend": backend}))
+    return {"iters": n, "seconds": dt, "iter_per_s": n / dt}
 
 
-def cpu_baseline(seconds: float):
-    env = dict(os.environ, MKL_NUM_THREADS="1", MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
-    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-baseline-only", "--cpu-seconds", str(seconds)],
-                       capture_output=True, text=True, env=env, timeout=600)
+def _cpu_disc_256(o):
+    """The 256^2 disc workload in full: HIO beta 0.9 for 200 iterations,
+    then ER for 20, static support, from the pattern's random-phase start."""
+    from libspimage_amd import workloads
+    p = workloads.disc_256(0)
+    ph = o.phaser(p.amplitudes.shape)
+    ph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
+    t0 = time.perf_counter()
+    ph.iterate(o.alg(1, 0, beta=[(0, 0.9)]), 200)
+    ph.iterate(o.alg(4, 0), 20)
+    dt = time.perf_counter() - t0
+    ef, fc = ph.efourier()
+    return {"seconds": dt, "final_efourier": ef, "fcfo": fc, "ereal": ph.ereal()}
+
+
+def _cpu_raar_1024(o):
+    """The 1024^2 reconstruction in full: RAAR beta 0.87 for 300 iterations
+    with an area shrinkwrap every 20 (blur 3.0 -> 0.7 px, 8 % of the pixels;
+    the reference's qsort select, src/support_update.c:136-151)."""
+    from libspimage_amd import workloads
+    from oracle.oracle import SUP_AREA
+    p = workloads.blob_1024()
+    ph = o.phaser(p.amplitudes.shape)
+    ph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
+    sup = o.sup(SUP_AREA, period=20, blur=[(0, 3.0), (300, 0.7)], values=[(0, 0.08)])
+    t0 = time.perf_counter()
+    ph.iterate(o.alg(2, 0, beta=[(0, 0.87)]), 300, sup)
+    dt = time.perf_counter() - t0
+    ef, fc = ph.efourier()
+    return {"seconds": dt, "iter_per_s": 300 / dt, "final_efourier": ef, "support_fraction": ph.support_fraction()}
+
+
+def _cpu_volume_256(o, dm_sample=5, er_sample=3):
+    """The 256^3 reconstruction, timed on a sample and extrapolated: DM
+    (beta 0.8, Elser's gammas) and ER iterations and one area shrinkwrap
+    (blur 3.0, 4 %), scaled to the run's 190 DM + 95 ER iterations and 15
+    updates (200 DM + 100 ER with an update every 20, each consuming one)."""
+    from libspimage_amd import workloads
+    p = workloads.ellipsoid_256()
+    ph = o.phaser(p.amplitudes.shape)
+    ph.set_inputs(p.amplitudes, p.mask.astype(bool), p.support.astype(bool), p.start)
+    g1, g2 = o.dm_gammas(0.8)
+    t0 = time.perf_counter()
+    ph.iterate(o.alg(3, 0, beta=[(0, 0.8)], gamma1=g1, gamma2=g2), dm_sample)
+    t_dm = (time.perf_counter() - t0) / dm_sample
+    t0 = time.perf_counter()
+    ph.iterate(o.alg(4, 0), er_sample)
+    t_er = (time.perf_counter() - t0) / er_sample
+    t0 = time.perf_counter()
+    ph.support_area_update(3.0, 0.04)
+    t_area = time.perf_counter() - t0
+    total = 190 * t_dm + 95 * t_er + 15 * t_area
+    return {"seconds_extrapolated": total, "dm_seconds_per_iteration": t_dm, "er_seconds_per_iteration": t_er,
+            "area_update_seconds": t_area,
+            "sample": f"{dm_sample} DM + {er_sample} ER iterations and 1 area update timed, extrapolated to "
+                      "190 DM + 95 ER + 15 updates"}
+
+
+def _accuracy(probe_dir):
+    """Each probe holds a state before one GPU iteration and the GPU's state
+    after it.  The same iteration in the FP64 oracle (its own FFT in double)
+    and in the FP32 oracle (the reference's CPU arithmetic with MKL's FFTW3
+    API) gives the GPU's relative L2 error against FP64, next to the
+    reference CPU path's own."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    out = {}
+    for f in sorted(Path(probe_dir).glob("*.npz")):
+        d = np.load(f)
+        kind = str(d["kind"])
+        beta = float(d["beta"])
+        ids = {"hio": 1, "raar": 2, "dm": 3, "er": 4}
+        res = {}
+        for prec in ("f64", "f32"):
+            o = Oracle(prec)
+            kw = {}
+            if kind == "dm":
+                kw = dict(zip(("gamma1", "gamma2"), o.dm_gammas(beta)))
+            alg = o.alg(ids[kind], 0, beta=[(0, beta)], **kw)
+            ph = o.phaser(d["amp"].shape)
+            ph.set_inputs(d["amp"], d["mask"].astype(bool), d["support"].astype(bool), d["before"])
+            ph.iterate(alg, 1)
+            res[prec] = np.array(ph.g1, np.complex128)
+            del ph
+        ref = res["f64"]
+        nrm = float(np.linalg.norm(ref))
+        out[f.stem] = {"iteration": kind, "shape": list(d["amp"].shape),
+                       "gpu_vs_fp64_rel_l2": float(np.linalg.norm(d["after"].astype(np.complex128) - ref) / nrm),
+                       "reference_cpu_fp32_vs_fp64_rel_l2": float(np.linalg.norm(res["f32"] - ref) / nrm)}
+    return out
+
+
+def cpu_baseline_worker(spec_json: str):
+    """Runs in a child process: the CPU tasks named in the spec."""
+    spec = json.loads(spec_json)
+    threads = int(spec.get("threads", 1))
+    o, backend = _cpu_oracle(threads)
+    res = {"threads": threads, "backend": backend}
+    for task in spec["tasks"]:
+        if task == "hio_1024":
+            res[task] = _cpu_hio_1024(o, float(spec.get("seconds", 12.0)))
+        elif task == "disc_256":
+            res[task] = _cpu_disc_256(o)
+        elif task == "raar_1024":
+            res[task] = _cpu_raar_1024(o)
+        elif task == "volume_256":
+            res[task] = _cpu_volume_256(o)
+        elif task == "accuracy":
+            res[task] = _accuracy(spec["probe_dir"])
+    print(json.dumps(res))
+
+
+def _run_cpu_child(spec: dict, threads: int):
+    env = dict(os.environ, MKL_NUM_THREADS=str(threads))
+    if threads == 1:
+        env.update(MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
+    else:
+        env.update(MKL_THREADING_LAYER="GNU", OMP_NUM_THREADS=str(threads))
+    spec = dict(spec, threads=threads)
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-baseline-only", "--cpu-spec", json.dumps(spec)],
+                       capture_output=True, text=True, env=env, timeout=900)
     if r.returncode != 0:
-        return {"value": None, "unit": "iter/s", "cores": 1, "kind": "port", "sample": f"failed: {r.stderr[-300:]}"}
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": d["iters"] / d["seconds"], "unit": "iter/s", "cores": 1, "kind": "port",
-            "sample": f"{d['iters']} HIO iterations of the 1024x1024 blob pattern in {d['seconds']:.1f} s "
-                      f"(oracle restatement of src/phasing.c:1142-1172, FFT: {d['backend']})"}
+        return {"error": r.stderr[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(seconds: float, probe_dir: str | None, volume: bool):
+    """The reference loop on the host cores: one thread (the reference's
+    default) and all threads of this process's CPU share, plus the accuracy
+    of each GPU result against the FP64 oracle."""
+    tasks1 = ["hio_1024", "disc_256", "raar_1024"] + (["volume_256"] if volume else [])
+    if probe_dir:
+        tasks1.append("accuracy")
+    one = _run_cpu_child({"tasks": tasks1, "seconds": seconds, "probe_dir": probe_dir}, 1)
+    nthr = _cpu_threads()
+    many = _run_cpu_child({"tasks": ["hio_1024", "raar_1024"] + (["volume_256"] if volume else []),
+                           "seconds": seconds}, nthr) if nthr > 1 else None
+    hio = one.get("hio_1024")
+    base = {"value": hio["iter_per_s"] if hio else None, "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": (f"{hio['iters']} HIO iterations of the 1024x1024 blob pattern in {hio['seconds']:.1f} s "
+                       f"(oracle restatement of src/phasing.c:1142-1172, FFT: {one.get('backend')})")
+            if hio else f"failed: {one.get('error')}",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "threads_all_cores": nthr}
+    if many and many.get("hio_1024"):
+        base["all_cores"] = {"value": many["hio_1024"]["iter_per_s"], "unit": "iter/s", "threads": nthr,
+                             "backend": many.get("backend"),
+                             "note": "FFTs threaded like the reference's sp_init_fft; the element-wise loops of "
+                                     "src/phasing.c stay serial as in the reference"}
+    workloads_cpu = {}
+    for key in ("disc_256", "raar_1024", "volume_256"):
+        entry = {}
+        if key in one:
+            entry["threads_1"] = one[key]
+        if many and key in many:
+            entry[f"threads_{nthr}"] = many[key]
+        if entry:
+            workloads_cpu[key] = entry
+    return base, workloads_cpu, one.get("accuracy"), {k: v for k, v in (("one", one), ("many", many))
+                                                     if v and "error" in v}
 
 
 def copy_calibration():
@@ -537,20 +825,47 @@ def main():
         copy_calibration()
         return
     if args.cpu_baseline_only:
-        cpu_baseline_worker(args.cpu_seconds)
+        cpu_baseline_worker(args.cpu_spec)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(independent reconstructions shard by id across GPUs with no exchange)", file=sys.stderr)
         raise SystemExit(2)
-    t_iter, roofline, it_roof, recon = gpu_bench(args)
-    batched = None if args.no_batched else batched_bench()
-    serving = None if args.no_serving else serving_bench()
-    volume = None if args.no_volume else volume_bench()
-    reconstructor = None if args.no_batched else reconstructor_bench()
-    prtf = None if args.no_batched else prtf_bench()
-    mixed = None if args.no_batched else mixed_radix_bench()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
+    import shutil
+    import tempfile
+    probe_dir = None if args.no_cpu_baseline else tempfile.mkdtemp(prefix="spx_accuracy_")
+    try:
+        t_iter, roofline, it_roof, recon = gpu_bench(args, probe_dir)
+        disc = disc_bench(probe_dir)
+        batched = None if args.no_batched else batched_bench()
+        serving = None if args.no_serving else serving_bench()
+        volume = None if args.no_volume else volume_bench(probe_dir=probe_dir)
+        reconstructor = None if args.no_batched else reconstructor_bench()
+        prtf = None if args.no_batched else prtf_bench()
+        mixed = None if args.no_batched else mixed_radix_bench()
+        cpu = cpu_workloads = accuracy = cpu_errors = None
+        if not args.no_cpu_baseline:
+            cpu, cpu_workloads, accuracy, cpu_errors = cpu_baseline(args.cpu_seconds, probe_dir, volume is not None)
+    finally:
+        if probe_dir:
+            shutil.rmtree(probe_dir, ignore_errors=True)
+    # GPU against CPU on the measured workloads (same inputs, same schedule)
+    if cpu_workloads:
+        gpu_side = {"disc_256": (disc, "seconds"), "raar_1024": (recon, "seconds"),
+                    "volume_256": (volume, "seconds")}
+        for key, entry in cpu_workloads.items():
+            g, field = gpu_side[key]
+            if not g:
+                continue
+            entry["gpu_seconds"] = g[field]
+            for tk, tv in list(entry.items()):
+                if not tk.startswith("threads_"):
+                    continue
+                cs = tv.get("seconds", tv.get("seconds_extrapolated"))
+                entry[f"speedup_vs_{tk}"] = cs / g[field] if cs else None
+                if "final_efourier" in tv and "final_efourier" in g:
+                    entry[f"final_efourier_rel_diff_vs_{tk}"] = abs(g["final_efourier"] - tv["final_efourier"]) / \
+                        max(abs(tv["final_efourier"]), 1e-30)
     out = {
         "metric": "HIO iter/s @1024^2 fp32 (%HBM roofline)",
         "value": 1.0 / t_iter,
@@ -569,6 +884,7 @@ def main():
         "roofline": roofline,
         "iteration_roofline": it_roof,
         "reconstruction_1024": recon,
+        "disc_256": disc,
         "batched": batched,
         "serving": serving,
         "volume_256": volume,
@@ -576,7 +892,11 @@ def main():
         "prtf": prtf,
         "mixed_radix_2d": mixed,
         "cpu_baseline": cpu,
+        "cpu_workloads": cpu_workloads,
+        "accuracy_vs_fp64": accuracy,
     }
+    if cpu_errors:
+        out["cpu_leg_errors"] = cpu_errors
     print(json.dumps(out))
 
 

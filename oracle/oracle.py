@@ -104,6 +104,8 @@ class Oracle:
         L.or_support_log.argtypes = [P, C.POINTER(C.c_int), C.c_int]
         L.or_set_fft_backend.restype = C.c_int
         L.or_set_fft_backend.argtypes = [C.c_int, C.c_char_p]
+        L.or_set_fft_threads.restype = C.c_int
+        L.or_set_fft_threads.argtypes = [C.c_int]
         cls._cache[precision] = self
         return self
 
@@ -135,6 +137,10 @@ class Oracle:
 
     def set_fft_backend(self, backend: int, libpath: str = "") -> int:
         return int(self.lib.or_set_fft_backend(backend, libpath.encode()))
+
+    def set_fft_threads(self, nthreads: int) -> int:
+        """Threads of the FFTW3-API library's plans (backend 1 only)."""
+        return int(self.lib.or_set_fft_threads(nthreads))
 
     def alg(self, type_: int, constraints: int = 0, beta=None, gamma1: float = 0.0, gamma2: float = 0.0):
         a = self.OrAlg()

@@ -164,6 +164,23 @@ EXPORT int or_set_fft_backend(int backend, const char *libpath) {
   return 0;
 }
 
+/* Threads of the FFTW3-API library's plans (the reference's sp_init_fft,
+ * src/fft.c:34-49: fftwf_init_threads + fftwf_plan_with_nthreads); only for
+ * the all-cores CPU baseline.  0 on success, -1 without a loaded library or
+ * thread support. */
+EXPORT int or_set_fft_threads(int nthreads) {
+  if (g_backend != 1) return -1;
+  typedef int (*init_fn)(void);
+  typedef void (*nthr_fn)(int);
+  void *h = dlopen(NULL, RTLD_NOW | RTLD_GLOBAL);
+  init_fn init = (init_fn)dlsym(h, "fftwf_init_threads");
+  nthr_fn with = (nthr_fn)dlsym(h, "fftwf_plan_with_nthreads");
+  if (!with) return -1;
+  if (init && !init()) return -1;
+  with(nthreads);
+  return 0;
+}
+
 /* unnormalised c2c over (nz, ny, nx), x fastest; sign -1 forward, +1 inverse */
 EXPORT void or_fft(const oc *in, oc *out, int nx, int ny, int nz, int sign) {
   const long N = (long)nx * ny * nz;

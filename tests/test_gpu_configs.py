@@ -148,8 +148,20 @@ def test_volume_256_one_iteration(sp, o32, ellipsoid, alg):
 
 def test_volume_256_area_update(sp, o32, ellipsoid):
     """Area shrinkwrap at 256^3 (blur 3.0 voxels, 4 % of the voxels) on the
-    same iterate: threshold within 1e-6 relative, mask bit-exact wherever
-    the oracle's |blur| is more than 1e-6 (relative) from the threshold."""
+    same iterate.
+
+    At 2^24 voxels the GPU's FP32 blur and the oracle's (a double-precision
+    FFT rounded to float) differ by about 2e-6 of the threshold, so the two
+    thresholds are compared through that difference: the threshold is the
+    k-th largest |blur|, and an order statistic moves by no more than the
+    largest change of any value, delta = max |blur_gpu - blur_oracle|.  So
+      * the GPU mask is its own blur thresholded at its own threshold,
+        bit-exact wherever that blur is more than 1e-6 (relative) from it;
+      * |t_gpu - t_oracle| <= delta, and delta <= 1e-5 max |blur| (the
+        transforms' FP32 error grows with the largest value, not with t);
+      * the masks agree wherever the oracle's blur is further than 2 delta
+        from the oracle's threshold (then both blurs fall on the same side
+        of both thresholds)."""
     from test_gpu_parity import _support_mask_matches
     p = ellipsoid
     shape = p.amplitudes.shape
@@ -163,23 +175,39 @@ def test_volume_256_area_update(sp, o32, ellipsoid):
         assert sp.spx_phaser_last_threshold(gph.ph, C.byref(tg)) == 0
         gph.ph.contents.iteration += 1  # refresh the getters' cache
         gmask = gph.get("support").real.astype(bool)
+    # the GPU's blur of the same magnitudes (the engine's transforms)
+    img = sp.image_from_array(np.abs(g).astype(np.complex64))
+    out = sp.sp_gaussian_blur(img, sigma)
+    assert bool(out), sp.last_error()
+    bg = np.abs(sp.image_view(out))
+    sp.sp_image_free(out)
+    sp.sp_image_free(img)
+    ok, amb = _support_mask_matches(gmask, bg > tg.value, bg, tg.value)
+    assert ok, f"GPU mask differs from its own blur away from its threshold ({amb} voxels within 1e-6)"
     oph = o32.phaser(shape)
     oph.set_inputs(p.amplitudes, meas, sup, g)
     to = oph.support_area_update(sigma, area)
-    assert abs(tg.value - to) <= 1e-6 * to, (tg.value, to)
     omask = (oph.flags & 1).astype(bool)
     del oph
     b = np.abs(o32.blur(np.abs(g).astype(np.complex64), sigma))
-    ok, amb = _support_mask_matches(gmask, omask, b, to)
-    assert ok, f"masks differ away from the threshold ({amb} voxels within 1e-6)"
+    delta = float(np.abs(bg - b).max())
+    bmax = float(b.max())
+    print(f"256^3 area update: thresholds {tg.value:.9g} (GPU) and {to:.9g} (oracle); blurs differ by <= {delta:.3g} "
+          f"(max |blur| {bmax:.4g})", flush=True)
+    assert delta <= 1e-5 * bmax, (delta, bmax)
+    assert abs(tg.value - to) <= delta, (tg.value, to, delta)
+    ok, amb = _support_mask_matches(gmask, omask, b, to, band=2 * delta)
+    assert ok, f"masks differ away from the threshold ({amb} voxels within 2 delta)"
     k = int(np.float32(b.size) * np.float32(area))
     assert abs(int(gmask.sum()) - k) <= amb + 1
 
 
 def _fft(sp, a, sign):
+    """sp_image_fft_fast / sp_image_ifft_fast (raw transforms: no phased /
+    shifted checks, which sp_image_ifft makes and aborts on)."""
     img = sp.image_from_array(np.ascontiguousarray(a, np.complex64))
-    out = sp.sp_image_fft(img) if sign < 0 else sp.sp_image_ifft(img)
-    assert bool(out), sp.last_error()
+    out = sp.image_from_array(np.zeros(a.shape, np.complex64))
+    (sp.sp_image_fft_fast if sign < 0 else sp.sp_image_ifft_fast)(img, out)
     r = sp.image_view(out).copy()
     sp.sp_image_free(out)
     sp.sp_image_free(img)
