@@ -260,19 +260,21 @@ def disc_bench(probe_dir=None):
     return out
 
 
-def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=2, seed=2000):
+def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=2, seed=2000, quantum=25):
     """Many-reconstruction workload: n_patterns seeded disc patterns with sizes
     drawn from {128, 256, 384, 512}^2, n_starts random-phase starts each, HIO
     beta 0.9 for `iters` iterations with an area shrinkwrap every `period`
     (blur 3.0 -> 0.7 px over the run, 8 % of the pixels).
 
     Batched: one SpxBatch per size class advances all its reconstructions
-    together (one launch per pass for the whole class).  Plain: the
-    reference-style loop, one SpPhaser per reconstruction, timed on
-    `plain_per_size` reconstructions per size class from the same starts and
-    extrapolated to the size class's count.  Inputs are uploaded before the
-    clock starts in both forms; the clock stops after a synchronising
-    efourier read-back."""
+    together (one launch per pass for the whole class).  The classes run
+    concurrently, each batch on its own stream, fed by one host loop in
+    quanta of `quantum` iterations; the same classes run one after another
+    for comparison.  Plain: the reference-style loop, one SpPhaser per
+    reconstruction, timed on `plain_per_size` reconstructions per size class
+    from the same starts and extrapolated to the size class's count.  Inputs
+    are uploaded before the clock starts in every form; the clock stops after
+    synchronising efourier read-backs."""
     import numpy as np
     from libspimage_amd import spimage as sp
     from libspimage_amd import workloads
@@ -286,33 +288,54 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     bmap, amap = sp.smap([(0, 3.0), (iters, 0.7)]), sp.smap([(0, 0.08)])
     keep += [bmap, amap]
     sarr = sp.sp_support_array_init(sp.sp_support_area_alloc(bmap, amap), period)
-    t_batch = t_plain = 0.0
-    per_size, worst = {}, 0.0
-    for n, pats in sorted(classes.items()):
-        nrec = len(pats) * n_starts
-        b = sp.Batch((n, n), nrec)
+
+    def build(n, pats):
+        b = sp.Batch((n, n), len(pats) * n_starts)
         for j, p in enumerate(pats):
-            for s in range(n_starts):
-                b.set_data(j * n_starts + s, p.amplitudes, p.mask, p.support)
+            for s_ in range(n_starts):
+                b.set_data(j * n_starts + s_, p.amplitudes, p.mask, p.support)
         b.init_model(sp.SpModelRandomPhases, seed + n)
         b.set_algorithm(hio)
         b.set_support_schedule(sarr)
         b.use_graphs(False)
-        starts = [b.model(r) for r in range(plain_per_size)]
+        return b
+
+    # serial classes
+    t_serial = 0.0
+    ef_serial, starts = {}, {}
+    for n, pats in sorted(classes.items()):
+        b = build(n, pats)
+        starts[n] = [b.model(r) for r in range(plain_per_size)]
         b.synchronize()
         t0 = time.perf_counter()
         b.iterate(iters)
-        ef_b, _ = b.efourier()
-        tb = time.perf_counter() - t0
+        ef_serial[n], _ = b.efourier()
+        t_serial += time.perf_counter() - t0
         b.close()
-        # plain loop over the first reconstructions of the class
+    # concurrent classes
+    bs = {n: build(n, pats) for n, pats in sorted(classes.items())}
+    for b in bs.values():
+        b.synchronize()
+    t0 = time.perf_counter()
+    for q in range(0, iters, quantum):
+        for b in bs.values():
+            b.iterate(min(quantum, iters - q))
+    ef_conc = {n: b.efourier()[0] for n, b in bs.items()}
+    t_conc = time.perf_counter() - t0
+    for b in bs.values():
+        b.close()
+    same = all(np.array_equal(ef_conc[n], ef_serial[n]) for n in classes)
+    # plain loop over the first reconstructions of each class
+    t_plain, per_size, worst = 0.0, {}, 0.0
+    for n, pats in sorted(classes.items()):
+        nrec = len(pats) * n_starts
         tp, ef_p = 0.0, []
         for r in range(plain_per_size):
             p = pats[r // n_starts]
             ph = sp.sp_phaser_alloc()
             if sp.sp_phaser_init(ph, hio, sarr, sp.SpEngineHIP) != 0:
                 raise RuntimeError(sp.last_error())
-            imgs = [sp.image_from_array(p.amplitudes, p.mask), sp.image_from_array(starts[r]),
+            imgs = [sp.image_from_array(p.amplitudes, p.mask), sp.image_from_array(starts[n][r]),
                     sp.image_from_array(p.support)]
             sp.sp_phaser_set_amplitudes(ph, imgs[0])
             if sp.sp_phaser_init_model(ph, imgs[1], 0) != 0 or sp.sp_phaser_init_support(ph, imgs[2], 0, 0.0) != 0:
@@ -324,19 +347,25 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             tp += time.perf_counter() - t0
             ef_p.append(ef)
             sp.sp_phaser_free(ph)
+            for im in imgs:
+                sp.sp_image_free(im)
         tp_class = tp / plain_per_size * nrec
-        diff = max(abs(float(ef_b[r]) - e) / max(abs(e), 1e-30) for r, e in enumerate(ef_p))
+        diff = max(abs(float(ef_serial[n][r]) - e) / max(abs(e), 1e-30) for r, e in enumerate(ef_p))
         worst = max(worst, diff)
-        t_batch += tb
         t_plain += tp_class
-        per_size[f"{n}x{n}"] = {"recons": nrec, "batched_s": tb, "plain_s_extrapolated": tp_class,
-                                "mean_efourier": float(np.mean(ef_b))}
+        per_size[f"{n}x{n}"] = {"recons": nrec, "plain_s_extrapolated": tp_class,
+                                "mean_efourier": float(np.mean(ef_serial[n]))}
     total = n_patterns * n_starts
     return {"workload": f"{n_patterns} patterns x {n_starts} random starts, sizes {{128,256,384,512}}^2, "
                         f"HIO beta 0.9, {iters} iterations, area shrinkwrap every {period}",
-            "recons": total, "batched_seconds": t_batch, "batched_recons_per_s": total / t_batch,
+            "recons": total, "batched_seconds": t_conc, "batched_recons_per_s": total / t_conc,
+            "batched_form": f"one batch per size class, the classes concurrently on their own streams "
+                            f"(quanta of {quantum} iterations)",
+            "serial_classes_seconds": t_serial, "serial_classes_recons_per_s": total / t_serial,
+            "concurrent_equals_serial": same,
             "plain_seconds_extrapolated": t_plain, "plain_recons_per_s": total / t_plain,
-            "speedup": t_plain / t_batch, "plain_sample": f"{plain_per_size} reconstructions per size class",
+            "speedup": t_plain / t_conc, "speedup_serial_classes": t_plain / t_serial,
+            "plain_sample": f"{plain_per_size} reconstructions per size class",
             "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
 
 
@@ -539,10 +568,10 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
     with ProcessPoolExecutor(workers, mp_context=get_context("spawn")) as ex:
         pats = dict(zip(shapes, ex.map(_serving_pattern, [(s, seed + i) for i, s in enumerate(shapes)])))
 
-    def requests(realtime=True):
+    def requests(realtime=True, keep=None):
         return [serve.Request(k, pats[s].amplitudes, pats[s].mask, pats[s].support, alg, its, sd,
                               beta=0.9 if alg == "hio" else 0.87, arrival=t if realtime else 0.0)
-                for k, (t, s, alg, its, sd) in enumerate(trace)]
+                for k, (t, s, alg, its, sd) in enumerate(trace) if keep is None or keep[k]]
 
     # warm-up: the tuner runs for every shape at every batch size the server
     # may create (Server.prepare), then every shape goes once through both
@@ -568,6 +597,14 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
     offline = srv.run(requests(False), realtime=False)
     t_off = time.perf_counter() - t0
     srv.close()
+    # below capacity: the same trace thinned at random to half its rate (a
+    # thinned Poisson process is a Poisson process of the lower rate)
+    keep = np.random.default_rng(seed + 1).random(len(trace)) < 0.5
+    srv = serve.Server()
+    t0 = time.perf_counter()
+    light = srv.run(requests(True, keep), realtime=True)
+    t_light = time.perf_counter() - t0
+    srv.close()
     plain_res, t_plain = plain.run(requests(True))
     ef_plain = {r.rid: r.efourier for r in plain_res}
     diff = max(abs(r.efourier - ef_plain[r.rid]) / max(abs(ef_plain[r.rid]), 1e-30) for r in live)
@@ -578,6 +615,9 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
             "requests": n, "shapes": len(shapes),
             "served": {"latency": serve.latency_summary(live), "makespan_s": t_live, "recons_per_s": n / t_live,
                        "batches_created": created_live},
+            "served_half_rate": {"offered_req_per_s": rate / 2, "requests": int(keep.sum()),
+                                 "latency": serve.latency_summary(light), "makespan_s": t_light,
+                                 "recons_per_s": int(keep.sum()) / t_light},
             "capacity_all_queued": {"seconds": t_off, "recons_per_s": n / t_off},
             "plain_loop": {"seconds": t_plain, "recons_per_s": n / t_plain,
                            "latency_fifo": serve.latency_summary(plain_res)},

@@ -10,9 +10,11 @@
  *
  * Reconstruction r of a batch of nrec holds its own amplitudes, measured
  * mask, support and iterates.  The phasing algorithm and the support
- * schedule (an SpSupportArray of area / threshold / static updates, with the
- * reference's update_period semantics) apply to all reconstructions; ER, HIO
- * and RAAR may be mixed per reconstruction with per-reconstruction beta.
+ * schedule (an SpSupportArray of area / threshold / centre-image / close /
+ * static updates, with the reference's update_period semantics) apply to
+ * all reconstructions, each update to each reconstruction's own support;
+ * ER, HIO and RAAR may be mixed per reconstruction with per-reconstruction
+ * beta.
  *
  * All functions return 0 on success and a negative code on error
  * (spx_last_error() holds the message).  Host arrays are in the reference's

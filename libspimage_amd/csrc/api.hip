@@ -971,97 +971,17 @@ int sp_support_close_update_support(SpSupportAlgorithm* alg, SpPhaser* ph) {
 
 // reference src/support_update.c:282-389 (sp_support_centre_image_cpu) with
 // sp_pixel_flags_translate_mask (src/image_util.c:3587-3611) and
-// sp_i3matrix_binary_center_of_mass_shifted (src/linear_alg.c:444-475).
+// sp_i3matrix_binary_center_of_mass_shifted (src/linear_alg.c:444-475), on
+// the device (Engine::support_centre_image): no download or upload.
 //
 // The support bit moves cyclically; the other flag bits stay in place.  The
 // reference also translates its cached copy of the model (ph->model), which
 // the next model read replaces because the update consumes an iteration;
 // the iterate itself does not move, so it does not move here either.
-//
-// The translations compose, so they are tracked as one offset o (support at
-// p = original support at p - o) and applied once.  The centre of mass is
-// summed in float in raster order, as the reference sums it, which is why
-// this update runs on the host: its result depends on that order once the
-// sums pass 2^24.  It costs one download and one upload of the N flag bytes.
 int sp_support_centre_image(SpSupportAlgorithm*, SpPhaser* ph) {
   Engine* e = engine_of(ph);
   if (!e) return -1;
-  const int nx = ph->nx, ny = ph->ny, nz = ph->nz;
-  const long n = ph->image_size;
-  std::vector<uint8_t> f((size_t)n);
-  if (e->download_flags(f.data())) {
-    report("sp_support_centre_image");
-    return -1;
-  }
-  int ox = 0, oy = 0, oz = 0;
-  auto wrap = [](long v, int m) { return (int)(((v % m) + m) % m); };
-  auto inside = [&](int x, int y, int z) {
-    const long i = ((long)wrap(z - oz, nz) * ny + wrap(y - oy, ny)) * nx + wrap(x - ox, nx);
-    return (f[i] & SpPixelInsideSupport) != 0;
-  };
-  auto translate = [&](int dx, int dy, int dz) {
-    ox = wrap((long)ox + dx, nx);
-    oy = wrap((long)oy + dy, ny);
-    oz = wrap((long)oz + dz, nz);
-  };
-  if (nz == 1) {  // SP_2D (the reference's 1-2 dimension branch)
-    for (int ix = 0; ix < nx; ix++)
-      if (inside(ix, (ny - 1) / 2, 0)) {
-        translate(0, (ny - 1) / 2, 0);
-        break;
-      }
-    for (int iy = 0; iy < ny; iy++)
-      if (inside((nx - 1) / 2, iy, 0)) {
-        translate((nx - 1) / 2, 0, 0);
-        break;
-      }
-  } else {
-    // each break leaves the inner loop only, so one scan may translate
-    // more than once, as in the reference
-    for (int iy = 0; iy < ny; iy++)
-      for (int iz = 0; iz < nz; iz++)
-        if (inside((nx - 1) / 2, iy, iz)) {
-          translate((nx - 1) / 2, 0, 0);
-          break;
-        }
-    for (int ix = 0; ix < nx; ix++)
-      for (int iz = 0; iz < nz; iz++)
-        if (inside(ix, (ny - 1) / 2, iz)) {
-          translate(0, (ny - 1) / 2, 0);
-          break;
-        }
-    for (int ix = 0; ix < nx; ix++)
-      for (int iy = 0; iy < ny; iy++)
-        if (inside(ix, iy, (nz - 1) / 2)) {
-          translate(0, 0, (nz - 1) / 2);
-          break;
-        }
-  }
-  float c[3] = {0.f, 0.f, 0.f};
-  int count = 0;
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++)
-        if (inside(x, y, z)) {
-          c[0] += (float)((x + (nx - 1) / 2) % nx);
-          c[1] += (float)((y + (ny - 1) / 2) % ny);
-          c[2] += (float)((z + (nz - 1) / 2) % nz);
-          count++;
-        }
-  if (count) {
-    for (int k = 0; k < 3; k++) c[k] /= (float)count;
-    c[0] -= (float)((nx - 1) / 2);
-    c[1] -= (float)((ny - 1) / 2);
-    c[2] -= (float)((nz - 1) / 2);
-  }
-  translate((int)(0.f - c[0]), (int)(0.f - c[1]), (int)(0.f - c[2]));
-  std::vector<uint8_t> out((size_t)n);
-  long i = 0;
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++, i++)
-        out[i] = (uint8_t)((f[i] & ~SpPixelInsideSupport) | (inside(x, y, z) ? SpPixelInsideSupport : 0));
-  if (e->upload_flags(out.data())) {
+  if (e->support_centre_image()) {
     report("sp_support_centre_image");
     return -1;
   }

@@ -95,8 +95,12 @@ int support_update(SpxBatch* b) {
       SpSupportThresholdParameters* p = (SpSupportThresholdParameters*)a->params;
       rc = b->e->shrinkwrap_threshold(spx_smap_interpolate(p->blur_radius_map, (real)b->iteration),
                                       spx_smap_interpolate(p->threshold, (real)b->iteration));
+    } else if (a->type == SpSupportCentreImage) {
+      rc = b->e->support_centre_image();
+    } else if (a->type == SpSupportClose) {
+      rc = b->e->support_close(((SpSupportCloseParameters*)a->params)->size);
     } else if (a->type != SpSupportStatic) {
-      return fail(-1, "batched support updates: area, threshold and static only");
+      return fail(-1, "batched support updates: area, threshold, centre image, close and static only");
     }
     if (rc) return rc;
   }

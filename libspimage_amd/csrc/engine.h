@@ -147,6 +147,10 @@ class Engine {
   // closing of the support: `size` dilations, then `size` erosions, with the
   // 6-neighbour cross (reference src/support_update.c:213-280)
   int support_close(int size);
+  // centre image: the support moved cyclically so that its centre of mass
+  // sits at the origin (reference src/support_update.c:282-389), for every
+  // reconstruction, on the device
+  int support_centre_image();
   // support = tmpl_abs > t (reference src/support_update.c:181-207).  The N
   // host values of tmpl_abs are uploaded on the first call with `key` and
   // kept on the device for the engine's lifetime.

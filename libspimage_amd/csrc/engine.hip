@@ -709,6 +709,219 @@ __global__ void k_close_pass(const uint8_t* __restrict__ src, uint8_t* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Centre-image support update (reference src/support_update.c:282-389 with
+// sp_pixel_flags_translate_mask, src/image_util.c:3587-3611, and
+// sp_i3matrix_binary_center_of_mass_shifted, src/linear_alg.c:444-475).
+// The support bit moves cyclically; the other flag bits stay.  Translations
+// compose into offsets o (support at p = original support at p - o).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap_to(long v, int m) { return (int)(((v % m) + m) % m); }
+
+// One work-group of 256 threads per reconstruction: the scans for support
+// on the middle row / column (2-D) or the middle planes (3-D), in the
+// reference's order.  In 3-D each break leaves the inner loop only, so a scan
+// may translate more than once, and later checks see the moved support.
+__global__ __launch_bounds__(256) void k_centre_offsets(const uint8_t* __restrict__ flags, int nx, int ny, int nz,
+                                                        long N, unsigned* ctl_all) {
+  const uint8_t* f = flags + (long)blockIdx.x * N;
+  const int hx = (nx - 1) / 2, hy = (ny - 1) / 2, hz = (nz - 1) / 2;
+  int ox = 0, oy = 0, oz = 0;
+  auto inside = [&](int x, int y, int z) {
+    return (f[((long)wrap_to(z - oz, nz) * ny + wrap_to(y - oy, ny)) * nx + wrap_to(x - ox, nx)] & kInsideSupport) != 0;
+  };
+  const int t = threadIdx.x;
+  if (nz == 1) {
+    bool any = false;
+    for (int ix = t; ix < nx; ix += 256) any |= inside(ix, hy, 0);
+    if (__syncthreads_or(any)) oy = wrap_to((long)oy + hy, ny);
+    any = false;
+    for (int iy = t; iy < ny; iy += 256) any |= inside(hx, iy, 0);
+    if (__syncthreads_or(any)) ox = wrap_to((long)ox + hx, nx);
+  } else {
+    for (int iy = 0; iy < ny; ++iy) {
+      bool any = false;
+      for (int iz = t; iz < nz; iz += 256) any |= inside(hx, iy, iz);
+      if (__syncthreads_or(any)) ox = wrap_to((long)ox + hx, nx);
+    }
+    for (int ix = 0; ix < nx; ++ix) {
+      bool any = false;
+      for (int iz = t; iz < nz; iz += 256) any |= inside(ix, hy, iz);
+      if (__syncthreads_or(any)) oy = wrap_to((long)oy + hy, ny);
+    }
+    for (int ix = 0; ix < nx; ++ix) {
+      bool any = false;
+      for (int iy = t; iy < ny; iy += 256) any |= inside(ix, iy, hz);
+      if (__syncthreads_or(any)) oz = wrap_to((long)oz + hz, nz);
+    }
+  }
+  if (t == 0) {
+    unsigned* c = ctl_all + (long)blockIdx.x * kCtlStride;
+    c[kCtlCentre] = (unsigned)ox;
+    c[kCtlCentre + 1] = (unsigned)oy;
+    c[kCtlCentre + 2] = (unsigned)oz;
+  }
+}
+
+// out[p] = support bit of src at p - o (mode 0: o = the middle-line offsets,
+// out one byte per pixel; mode 1: o = the centre-of-mass offsets, out = the
+// flags with their support bit replaced)
+__global__ void k_centre_move(const uint8_t* __restrict__ src, uint8_t* __restrict__ out, int nx, int ny, int nz, long N,
+                              long n, const unsigned* __restrict__ ctl_all, int mode) {
+  GRID_STRIDE(i, n) {
+    const long rec = i / N, j = i - rec * N;
+    const int x = (int)(j % nx);
+    const long r = j / nx;
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    const unsigned* c = ctl_all + rec * kCtlStride;
+    int ox, oy, oz;
+    if (mode == 0) {
+      ox = (int)c[kCtlCentre];
+      oy = (int)c[kCtlCentre + 1];
+      oz = (int)c[kCtlCentre + 2];
+    } else {
+      ox = (int)(c[kCtlCentreShift] & 0xFFFFu);
+      oy = (int)(c[kCtlCentreShift] >> 16);
+      oz = (int)c[kCtlCentreShift + 1];
+    }
+    const long sidx = rec * N + ((long)wrap_to(z - oz, nz) * ny + wrap_to(y - oy, ny)) * nx + wrap_to(x - ox, nx);
+    const uint8_t bit = src[sidx] & kInsideSupport;
+    out[i] = mode == 0 ? bit : (uint8_t)((out[i] & ~kInsideSupport) | bit);
+  }
+}
+
+// One work-group of 256 threads per reconstruction: the support's centre of
+// mass in the translated frame, summed in float in raster order as the
+// reference sums it (src/linear_alg.c:444-475), and the translation that
+// moves it to the origin.  The sums are of non-negative integers, so float
+// partial sums are exact up to 2^24: chunks of 4096 pixels whose exact total
+// keeps a sum within 2^24 are added at once; from the first chunk that would
+// pass it, that sum is accumulated pixel by pixel in raster order (one
+// thread over the chunk's support pixels, compacted into LDS in order).
+__global__ __launch_bounds__(256) void k_centre_mass(const uint8_t* __restrict__ sup, int nx, int ny, int nz, long N,
+                                                     unsigned* ctl_all) {
+  constexpr int PER = 16, CH = 256 * PER;
+  __shared__ ushort4 list[CH];
+  __shared__ int wsum[4][4];
+  __shared__ int wpre[4];
+  __shared__ int need_list;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint8_t* f = sup + (long)blockIdx.x * N;
+  const int hx = (nx - 1) / 2, hy = (ny - 1) / 2, hz = (nz - 1) / 2;
+  const long lim = 1L << 24;
+  // thread 0's running state
+  float s[3] = {0.f, 0.f, 0.f};
+  long ex[3] = {0, 0, 0};
+  bool seq[3] = {false, false, false};
+  long count = 0;
+  for (long c0 = 0; c0 < N; c0 += CH) {
+    const long p0 = c0 + (long)t * PER;
+    int x = (int)(p0 % nx);
+    const long r = p0 / nx;
+    int y = (int)(r % ny), z = (int)(r / ny);
+    int cnt = 0, sx = 0, sy = 0, sz = 0;
+    unsigned short cx[PER], cy[PER], cz[PER];
+    unsigned in_mask = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const long p = p0 + j;
+      const bool in = p < N && (f[p < N ? p : N - 1] & kInsideSupport);
+      cx[j] = (unsigned short)((x + hx) % nx);
+      cy[j] = (unsigned short)((y + hy) % ny);
+      cz[j] = (unsigned short)((z + hz) % nz);
+      if (in) {
+        in_mask |= 1u << j;
+        ++cnt;
+        sx += cx[j];
+        sy += cy[j];
+        sz += cz[j];
+      }
+      if (++x == nx) {
+        x = 0;
+        if (++y == ny) {
+          y = 0;
+          ++z;
+        }
+      }
+    }
+    // per-wave inclusive scan of cnt and sums of the four values
+    int inc = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += v;
+    }
+    int v4[4] = {cnt, sx, sy, sz};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v4[q] += __shfl_xor(v4[q], off, 64);
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) wsum[w][q] = v4[q];
+    __syncthreads();
+    if (t == 0) {
+      int tot[4] = {0, 0, 0, 0};
+      int pre = 0;
+      for (int k = 0; k < 4; ++k) {
+        wpre[k] = pre;
+        pre += wsum[k][0];
+        for (int q = 0; q < 4; ++q) tot[q] += wsum[k][q];
+      }
+      int need = 0;
+      for (int q = 0; q < 3; ++q) {
+        if (!seq[q] && ex[q] + tot[q + 1] <= lim) {
+          ex[q] += tot[q + 1];
+        } else {
+          if (!seq[q]) {
+            seq[q] = true;
+            s[q] = (float)ex[q];  // exact: ex[q] <= 2^24
+          }
+          need = 1;
+        }
+      }
+      count += tot[0];
+      wsum[0][0] = tot[0];
+      need_list = need;
+    }
+    __syncthreads();
+    if (need_list) {
+      int o = wpre[w] + inc - cnt;  // exclusive offset of this thread's pixels
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        if (in_mask & (1u << j)) list[o++] = make_ushort4(cx[j], cy[j], cz[j], 0);
+      __syncthreads();
+      if (t == 0) {
+        const int n = wsum[0][0];
+        for (int e = 0; e < n; ++e) {
+          const ushort4 v = list[e];
+          if (seq[0]) s[0] += (float)v.x;
+          if (seq[1]) s[1] += (float)v.y;
+          if (seq[2]) s[2] += (float)v.z;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int q = 0; q < 3; ++q)
+      if (!seq[q]) s[q] = (float)ex[q];
+    int d[3] = {0, 0, 0};
+    if (count) {
+      const float h[3] = {(float)hx, (float)hy, (float)hz};
+      for (int q = 0; q < 3; ++q) {
+        float c = s[q] / (float)count;
+        c -= h[q];
+        d[q] = (int)(0.f - c);
+      }
+    }
+    unsigned* c = ctl_all + (long)blockIdx.x * kCtlStride;
+    const int dx = wrap_to(d[0], nx), dy = wrap_to(d[1], ny), dz = wrap_to(d[2], nz);
+    c[kCtlCentreShift] = (unsigned)dx | ((unsigned)dy << 16);
+    c[kCtlCentreShift + 1] = (unsigned)dz;
+  }
+}
+
 // support = tmpl[i] > t for every reconstruction (tmpl holds one of N)
 // (reference src/support_update.c:196-200 with :392-400)
 __global__ void k_template_mask(const float* __restrict__ tmpl, uint8_t* flags, long N, long n, float t) {
@@ -1588,6 +1801,24 @@ int Engine::support_close(int size) {
   return 0;
 }
 
+// The centre-image update on the device: the middle-line scans, the
+// support moved by them into the byte view of `absb`, its centre of mass,
+// and the flags' support bits replaced by that copy moved to the origin.
+int Engine::support_centre_image() {
+  if (nx > 65535 || ny > 65535) {
+    set_error("centre image: axis lengths above 65535");
+    return -1;
+  }
+  uint8_t* tmp = reinterpret_cast<uint8_t*>(absb);
+  hipLaunchKernelGGL(k_centre_offsets, dim3(nrec), dim3(256), 0, s, flags, nx, ny, nz, N, ctl);
+  hipLaunchKernelGGL(k_centre_move, ew_grid(NT), dim3(256), 0, s, flags, tmp, nx, ny, nz, N, NT, ctl, 0);
+  hipLaunchKernelGGL(k_centre_mass, dim3(nrec), dim3(256), 0, s, tmp, nx, ny, nz, N, ctl);
+  hipLaunchKernelGGL(k_centre_move, ew_grid(NT), dim3(256), 0, s, tmp, flags, nx, ny, nz, N, NT, ctl, 1);
+  SPX_HIP(hipGetLastError());
+  ++flags_version;
+  return 0;
+}
+
 int Engine::support_template(unsigned long long key, const float* tmpl_abs, float t) {
   float* d = nullptr;
   for (auto& e : templates)
@@ -1703,5 +1934,6 @@ int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
   SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
+
 
 }  // namespace spx

@@ -117,7 +117,13 @@ constexpr int kHistSlices = 8;
 constexpr int kHistLevel2 = kHistSlices * 2048, kHistLevel3 = kHistLevel2 + 2048;
 constexpr int kHistStride = kHistLevel3 + 2048;
 constexpr int kCtlStride = 16;
-enum CtlSlot { kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlMax = 8 /* .. 15, one per XCD */ };
+// The centre-image update keeps its offsets there too: the middle-line
+// translation (x, y, z in slots 3..5) and the centre-of-mass translation
+// (x | y << 16 in slot 6, z in slot 7), each wrapped into [0, n).
+enum CtlSlot {
+  kCtlK = 0, kCtlPrefix = 1, kCtlThresh = 2, kCtlCentre = 3 /* .. 5 */, kCtlCentreShift = 6 /* .. 7 */,
+  kCtlMax = 8 /* .. 15, one per XCD */
+};
 
 struct RowArgs {
   const float2* in;
@@ -227,6 +233,26 @@ __device__ __forceinline__ float2 apply_constraints(float2 g, int c) {
   }
   return g;
 }
+
+// apply_constraints with the choice among the constraint sets made once per
+// thread (the flags are uniform), so the per-pixel code has no branches
+struct SupportConstraints {
+  bool zero_im, clamp_re, clamp_im, flip;
+  __device__ explicit SupportConstraints(int c) {
+    const bool real = c & kRealObject;
+    const bool posr = !real && (c & kPositiveRealObject);
+    const bool posc = !real && !posr && (c & kPositiveComplexObject);
+    zero_im = real || posr;
+    clamp_re = posr || posc;
+    clamp_im = posc;
+    flip = c & kPositivityFlipping;
+  }
+  __device__ __forceinline__ float2 apply(float2 g) const {
+    const float x = (clamp_re && g.x < 0.f) ? (flip ? fabsf(g.x) : 0.f) : g.x;
+    const float y = zero_im ? 0.f : ((clamp_im && g.y < 0.f) ? (flip ? fabsf(g.y) : 0.f) : g.y);
+    return make_float2(x, y);
+  }
+};
 
 // Block-wide sum of NV doubles; result valid in thread 0.
 template <int NV, int NT>
@@ -439,74 +465,101 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
     else return x;
   };
 
-  float2 v[NR], vf[DM ? NR : 1], g0v[NR];
-  uint8_t fl[NR];
-#pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    v[q] = ld_line(&a.W[wbase + wpos(in_pos<L, E>(t, q))], a.nt);
-    if constexpr (DM) vf[q] = ld_line(&a.Wf[wbase + wpos(in_pos<L, E>(t, q))], a.nt);
-  }
-#pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const long idx = base + out_pos<L, E>(t, q);
-    g0v[q] = (a.nt & 4) ? ld_nt(&a.gcur[idx]) : a.gcur[idx];
-    fl[q] = a.flags[idx];
-  }
+  // Operands in the order they are needed: the twiddles and W for the
+  // inverse transform, then g_k, the flags and the per-reconstruction
+  // parameters for the update, which arrive during the transform.  Uniform
+  // choices (cache hints, optional outputs) are branched on outside the
+  // per-element loops.
   TwRegs<L, E> twr;
   load_tw<L, E>(a.tw, t, twr);
+  float2 v[NR], vf[DM ? NR : 1], g0v[NR];
+  uint8_t fl[NR];
+  if (a.nt & 1) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      v[q] = ld_nt(&a.W[wbase + wpos(in_pos<L, E>(t, q))]);
+      if constexpr (DM) vf[q] = ld_nt(&a.Wf[wbase + wpos(in_pos<L, E>(t, q))]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      v[q] = a.W[wbase + wpos(in_pos<L, E>(t, q))];
+      if constexpr (DM) vf[q] = a.Wf[wbase + wpos(in_pos<L, E>(t, q))];
+    }
+  }
+  if (a.nt & 4) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) g0v[q] = ld_nt(&a.gcur[base + out_pos<L, E>(t, q)]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) g0v[q] = a.gcur[base + out_pos<L, E>(t, q)];
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) fl[q] = a.flags[base + out_pos<L, E>(t, q)];
+  const int rec = act ? line / a.lines_per_rec : 0;
+  const float beta = a.beta_rec ? a.beta_rec[rec] : a.beta;
+  const int alg = a.alg_rec ? a.alg_rec[rec] : a.alg;
+
   fft_line<L, E, +1>(v, va, vb, t, twr);
   if constexpr (DM) {
     __syncthreads();
     fft_line<L, E, +1>(vf, va, vb, t, twr);
   }
 
-  const int rec = act ? line / a.lines_per_rec : 0;
-  const float beta = a.beta_rec ? a.beta_rec[rec] : a.beta;
-  const int alg = a.alg_rec ? a.alg_rec[rec] : a.alg;
   // outside the support g1 = cp * gp + cg * g0: HIO g0 - beta gp
   // (src/phasing.c:1160-1168), RAAR beta g0 + (1 - 2 beta) gp
   // (src/phasing.c:1194-1215), ER 0 (src/phasing.c:1128-1136); inside, gp
   const bool er = !(alg == kHIO || alg == kRAAR);
   const float cp = alg == kHIO ? -beta : 1.f - 2.f * beta;
   const float cg = alg == kHIO ? 1.f : beta;
+  const SupportConstraints sc(a.constraints);
+  float2 g1v[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    const long idx = base + out_pos<L, E>(t, q);
     const float2 P = v[q];
     const float2 g0 = g0v[q];
     const bool in = fl[q] & kInsideSupport;
     float2 g1;
     if constexpr (!DM) {
       const float2 gp = make_float2(P.x * a.scale, P.y * a.scale);
-      if (a.gp && act) a.gp[idx] = gp;
       float2 o = make_float2(fmaf(cp, gp.x, cg * g0.x), fmaf(cp, gp.y, cg * g0.y));
-      if (er) o = make_float2(0.f, 0.f);
-      const float2 gi = apply_constraints(gp, a.constraints);
+      o = er ? make_float2(0.f, 0.f) : o;
+      const float2 gi = sc.apply(gp);
       g1 = make_float2(in ? gi.x : o.x, in ? gi.y : o.y);
     } else {
       // src/phasing.c:1243-1252; gp stays unscaled like the CPU engine
-      if (a.gp && act) a.gp[idx] = P;
       const float2 Pf = vf[q];
-      if (in) {
-        g1.x = g0.x + beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x);
-        g1.y = g0.y + beta * ((1.f + a.gamma2) * P.y / a.nsize - a.gamma2 * g0.y);
-      } else {
-        g1 = g0;
-      }
+      const float2 gin = make_float2(g0.x + beta * ((1.f + a.gamma2) * P.x / a.nsize - a.gamma2 * g0.x),
+                                     g0.y + beta * ((1.f + a.gamma2) * P.y / a.nsize - a.gamma2 * g0.y));
+      g1 = in ? gin : g0;
       g1.x -= beta * Pf.x / a.nsize;
       g1.y -= beta * Pf.y / a.nsize;
-      if (in) g1 = apply_constraints(g1, a.constraints);
+      const float2 gc = sc.apply(g1);
+      g1 = in ? gc : g1;
+      // f1 of the next iteration (src/phasing.c:1260-1274)
+      vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
     }
-    if (act) {
-      if (a.nt & 8) st_nt(&a.gnew[idx], g1);
-      else a.gnew[idx] = g1;
-    } else if (idle_line) {
-      a.gnew[idx] = g0;
-    }
-    v[q] = g1;
-    // f1 of the next iteration (src/phasing.c:1260-1274)
-    if constexpr (DM) vf[q] = in ? g1 : make_float2(-a.gamma1 * g1.x, -a.gamma1 * g1.y);
+    g1v[q] = g1;
   }
+  if (a.gp && act) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      a.gp[base + out_pos<L, E>(t, q)] = DM ? v[q] : make_float2(v[q].x * a.scale, v[q].y * a.scale);
+  }
+  if (act) {
+    if (a.nt & 8) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) st_nt(&a.gnew[base + out_pos<L, E>(t, q)], g1v[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) a.gnew[base + out_pos<L, E>(t, q)] = g1v[q];
+    }
+  } else if (idle_line) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) a.gnew[base + out_pos<L, E>(t, q)] = g0v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = g1v[q];
   out_to_in<L, E>(v, va, t);
   if constexpr (DM) out_to_in<L, E>(vf, va, t);
   __syncthreads();
@@ -516,10 +569,18 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
     fft_line<L, E, -1>(vf, va, vb, t, twr);
   }
   if (act) {
+    if (a.nt & 2) {
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      st_line(&a.W[wbase + wpos(out_pos<L, E>(t, q))], v[q], a.nt);
-      if constexpr (DM) st_line(&a.Wf[wbase + wpos(out_pos<L, E>(t, q))], vf[q], a.nt);
+      for (int q = 0; q < NR; ++q) {
+        st_nt(&a.W[wbase + wpos(out_pos<L, E>(t, q))], v[q]);
+        if constexpr (DM) st_nt(&a.Wf[wbase + wpos(out_pos<L, E>(t, q))], vf[q]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        a.W[wbase + wpos(out_pos<L, E>(t, q))] = v[q];
+        if constexpr (DM) a.Wf[wbase + wpos(out_pos<L, E>(t, q))] = vf[q];
+      }
     }
   }
 }
@@ -611,10 +672,18 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
   const long st = a.geo.axis_stride;
   const View va{shA, c}, vb{shB, c};
 
+  // twiddles and line data first, then the amplitudes of the projection,
+  // which arrive during the forward transform
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
   float2 v[NR];
+  if (a.nt & 1) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q) v[q] = ld_line(&a.in[base + in_pos<L, E>(t, q) * st], a.nt);
-  // amplitudes of the projection, fetched with the data
+    for (int q = 0; q < NR; ++q) v[q] = ld_nt(&a.in[base + in_pos<L, E>(t, q) * st]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q] = a.in[base + in_pos<L, E>(t, q) * st];
+  }
   float ampv[(MODE == kColProject || MODE == kColEFourier) ? NR : 1];
   if constexpr (MODE == kColProject || MODE == kColEFourier) {
     if (MODE == kColEFourier || a.objective == 0) {
@@ -622,27 +691,48 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
       for (int q = 0; q < NR; ++q) ampv[q] = a.ampx[base + out_pos<L, E>(t, q) * st];
     }
   }
-  TwRegs<L, E> twr;
-  load_tw<L, E>(a.tw, t, twr);
   if constexpr (MODE == kColInv) fft_line<L, E, +1>(v, va, vb, t, twr);
   else fft_line<L, E, -1>(v, va, vb, t, twr);
 
   if constexpr (MODE == kColProject) {
+    if (a.objective == 0 && !a.margin) {
+      // the common case, branch-free: the modulus projection of
+      // project_pixel without margins (src/phasing.c:1070-1100)
+      const bool centro = a.centro != 0;
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      if (!act) continue;
-      const long idx = base + out_pos<L, E>(t, q) * st;
-      float2 G = v[q];
-      if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
-      if (a.objective == 0) {
-        G = project_pixel(G, ampv[q], a, idx);
-      } else {
-        // src/phasing.c:1103-1109: measured pixels take the phased
-        // amplitude; unmeasured entries carry kUnmeasuredBits in .x
-        const float2 P = a.phased_amp[idx];
-        if (__float_as_uint(P.x) != kUnmeasuredBits) G = P;
+      for (int q = 0; q < NR; ++q) {
+        float2 G = v[q];
+        G.y = centro ? 0.f : G.y;  // src/phasing.c:1061-1068
+        const float amp = ampv[q];
+        // m = amp / |G| as amp * rsqrt(|G|^2): 1-2 ulp from the correctly
+        // rounded quotient, a third of its instructions.  A denormal |G|^2 is
+        // scaled by 2^64 first (exactly), as the reciprocal square root
+        // flushes denormal inputs; |G|^2 = 0 still gives an infinite m.
+        const float sq = G.y * G.y + G.x * G.x;
+        const bool tiny = sq < 1.17549435e-38f;
+        const float m = amp * rsqrtf(tiny ? sq * 0x1p64f : sq) * (tiny ? 0x1p32f : 1.f);
+        const bool fin = isfinite(m);
+        const float2 P = make_float2(fin ? G.x * m : amp, fin ? G.y * m : 0.f);
+        const bool unm = __float_as_uint(amp) == kUnmeasuredBits;
+        v[q] = make_float2(unm ? G.x : P.x, unm ? G.y : P.y);
       }
-      v[q] = G;
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        if (!act) continue;
+        const long idx = base + out_pos<L, E>(t, q) * st;
+        float2 G = v[q];
+        if (a.centro) G.y = 0.f;  // src/phasing.c:1061-1068
+        if (a.objective == 0) {
+          G = project_pixel(G, ampv[q], a, idx);
+        } else {
+          // src/phasing.c:1103-1109: measured pixels take the phased
+          // amplitude; unmeasured entries carry kUnmeasuredBits in .x
+          const float2 P = a.phased_amp[idx];
+          if (__float_as_uint(P.x) != kUnmeasuredBits) G = P;
+        }
+        v[q] = G;
+      }
     }
     out_to_in<L, E>(v, va, t);
     __syncthreads();
@@ -705,9 +795,15 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
     return;
   }
 
+  if (act) {
+    if (a.nt & 2) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q)
-    if (act) st_line(&a.W[base + out_pos<L, E>(t, q) * st], v[q], a.nt);
+      for (int q = 0; q < NR; ++q) st_nt(&a.W[base + out_pos<L, E>(t, q) * st], v[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) a.W[base + out_pos<L, E>(t, q) * st] = v[q];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

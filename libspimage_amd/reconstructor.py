@@ -20,9 +20,12 @@ Differences from the reference:
   work in it);
 * arrays of ``reconstruct_loop`` are (n, y, x) / (n, z, y, x) also for
   non-square shapes (the reference allocates (n, x, y),
-  src/_spimage_reconstructor.py:683-685);
-* ``center_image=True`` is refused: the centre-image support update is not
-  provided by this engine.
+  src/_spimage_reconstructor.py:683-685).
+
+``center_image=True`` appends the centre-image update to the support
+algorithm's array, as the reference does (src/_spimage_reconstructor.py:
+480-484); it runs on the device after every support update, for every
+reconstruction of a loop.
 """
 from __future__ import annotations
 
@@ -128,10 +131,7 @@ class Reconstructor:
             logger.error("append_support_algorithm accepts algorithms of the following types: %s",
                          list(_SUPPORT_KWARGS))
             return
-        if center_image:
-            logger.error("center_image is not provided by this engine")
-            return
-        conf = {"type": type, "number_of_iterations": number_of_iterations}
+        conf = {"type": type, "number_of_iterations": number_of_iterations, "center_image": bool(center_image)}
         for k in _SUPPORT_KWARGS[type]:
             if k not in kwargs:
                 logger.error("support algorithm %s requires the argument %s", type, k)
@@ -221,6 +221,8 @@ class Reconstructor:
                 arr = sp.sp_support_array_init(alloc(blur, val), conf["update_period"])
             else:
                 arr = sp.sp_support_array_init(sp.sp_support_static_alloc(), n)
+            if conf.get("center_image"):
+                sp.sp_support_array_append(arr, sp.sp_support_centre_image_alloc())
             supports.append((n, arr))
             i += n
         phasings, i = [], 0

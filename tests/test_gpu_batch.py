@@ -75,6 +75,30 @@ def test_batch_shrinkwrap_equals_single(sp):
     b.close()
 
 
+@pytest.mark.parametrize("shape", [(128, 128), (24, 20, 16)])
+def test_batch_centre_image_and_close_equal_single(sp, shape):
+    """Area shrinkwrap followed by the centre-image and closing updates, each
+    applied to every reconstruction's own support on the device, equals the
+    same support array on single phasers."""
+    pats = patterns(shape, 3, seed=70)
+
+    def array():
+        a = sp.sp_support_array_init(sp.sp_support_area_alloc(sp.smap([(0, 2.0)]), sp.smap([(0, 0.12)])), 5)
+        sp.sp_support_array_append(a, sp.sp_support_centre_image_alloc())
+        sp.sp_support_array_append(a, sp.sp_support_close_alloc(1))
+        return a
+
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio, array())
+    b.iterate(15)
+    got, gsup = b.model(), b.support()
+    for r, p in enumerate(pats):
+        m, s, _ = run_single(sp, p, "hio", 15, sup_array=array())
+        assert np.array_equal(gsup[r], s), r
+        assert np.array_equal(got[r], m), r
+    b.close()
+
+
 def test_per_reconstruction_algorithms(sp):
     pats = patterns((64, 64), 3, seed=60)
     algs = np.array([4, 1, 2], np.int32)  # ER, HIO, RAAR

@@ -88,14 +88,17 @@ def test_centre_image(sp, o32, shape, centre, radius):
     assert np.array_equal(model, g)
 
 
-def test_centre_image_float_sums_past_2_24(sp, o32):
+@pytest.mark.parametrize("rect", [(94, 102, 677, 775), (180, 40, 551, 431), (100, 50, 600, 880)])
+def test_centre_image_float_sums_past_2_24(sp, o32, rect):
     """Half of a 1024^2 image in the support: the centre-of-mass sums pass
     2^24, where float addition rounds and the order of the sum decides the
-    result; the engine sums in the reference's raster order."""
+    result; the engine sums in the reference's raster order.  For the first
+    two rectangles the float32 raster-order result is one pixel away from the
+    exact centre of mass (tests/test_oracle_support_shapes.py pins this)."""
     shape = (1024, 1024)
+    y0, x0, h, w = rect
     sup = np.zeros(shape, bool)
-    sup[100:700, 50:930] = True
-    sup[300:310, :] = False
+    sup[y0:y0 + h, x0:x0 + w] = True
     amp, meas, g = _inputs(shape, 3)
     sarr = sp.sp_support_array_init(sp.sp_support_centre_image_alloc(), 1)
     with GpuPhaser(sp, "er", amp, meas, sup, g, sup_array=sarr) as gph:
@@ -103,6 +106,24 @@ def test_centre_image_float_sums_past_2_24(sp, o32):
     oph = _oracle(o32, amp, meas, sup, g)
     oph.support_centre_image()
     assert np.array_equal(got, (oph.flags & 1).astype(bool))
+
+
+def test_centre_image_volume_sums_past_2_24(sp, o32):
+    """A 128^3 volume with half of it in the support: the z and y sums pass
+    2^24 and many 4096-voxel chunks are summed pixel by pixel."""
+    shape = (128, 128, 128)
+    sup = np.zeros(shape, bool)
+    sup[10:90, 20:110, 5:100] = True
+    sup[40:44, :, :] = False
+    amp, meas, g = _inputs(shape, 4)
+    sarr = sp.sp_support_array_init(sp.sp_support_centre_image_alloc(), 1)
+    with GpuPhaser(sp, "er", amp, meas, sup, g, sup_array=sarr) as gph:
+        got = _update_once(sp, gph, sarr)
+    oph = _oracle(o32, amp, meas, sup, g)
+    oph.support_centre_image()
+    want = (oph.flags & 1).astype(bool)
+    assert not np.array_equal(want, sup)
+    assert np.array_equal(got, want)
 
 
 def _cabs32(z):

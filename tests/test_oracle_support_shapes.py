@@ -138,6 +138,45 @@ def test_centre_image_half_move_matters(o32):
     assert np.all(np.abs(_shifted_com((ph.flags & 1).astype(bool))) < 1.0)
 
 
+def _raster_float32_centre_shift(sup):
+    """The translation the reference's 2-D centre-image update applies
+    (src/support_update.c:304-389), restated in numpy: the middle-row and
+    middle-column moves, then the centre of mass of the moved support with
+    each coordinate summed sequentially in float32 in raster order
+    (src/linear_alg.c:444-475; np.add.accumulate is a sequential float32
+    loop), divided by the count in float32, truncated toward zero."""
+    ny, nx = sup.shape
+    hy, hx = (ny - 1) // 2, (nx - 1) // 2
+    oy = hy if sup[hy, :].any() else 0
+    ox = hx if sup[:, hx].any() else 0
+    moved = np.roll(sup, (oy, ox), axis=(0, 1))
+    ys, xs = np.nonzero(moved)  # raster order: y, then x
+    sx = np.add.accumulate(((xs + hx) % nx).astype(np.float32))[-1]
+    sy = np.add.accumulate(((ys + hy) % ny).astype(np.float32))[-1]
+    cnt = np.float32(len(xs))
+    cx = np.float32(sx / cnt) - np.float32(hx)
+    cy = np.float32(sy / cnt) - np.float32(hy)
+    exact_x = ((xs + hx) % nx).sum() / len(xs) - hx
+    exact_y = ((ys + hy) % ny).sum() / len(ys) - hy
+    return (oy + int(-cy), ox + int(-cx)), (oy + int(-exact_y), ox + int(-exact_x))
+
+
+def test_centre_image_float32_raster_order(o32):
+    """A support whose coordinate sums pass 2^24: summed in float32 in raster
+    order (as the reference sums them) the centre of mass lands one pixel
+    away from the exact one, and the oracle moves the support by the float32
+    result."""
+    shape = (1024, 1024)
+    for y0, x0, h, w in ((94, 102, 677, 775), (180, 40, 551, 431)):
+        sup = np.zeros(shape, bool)
+        sup[y0:y0 + h, x0:x0 + w] = True
+        (dy, dx), exact = _raster_float32_centre_shift(sup)
+        assert (dy, dx) != exact  # the summation order decides this case
+        ph, _ = _phaser(o32, sup)
+        ph.support_centre_image()
+        assert np.array_equal((ph.flags & 1).astype(bool), np.roll(sup, (dy, dx), axis=(0, 1)))
+
+
 def test_centre_image_empty_support(o32):
     sup = np.zeros((16, 16), bool)
     ph, _ = _phaser(o32, sup)
