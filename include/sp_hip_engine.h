@@ -59,6 +59,18 @@ spimage_EXPORT int spx_phaser_tiled(SpPhaser *ph);
  * Runs on the library's stream and returns after completion. */
 spimage_EXPORT int spx_fft_device(void *d_in, void *d_out, int nx, int ny, int nz, int direction);
 
+/* The unnormalised transform of sp_image_fft_fast / sp_image_ifft_fast
+ * (direction -1 / +1) on a host array of nx*ny*nz interleaved complex
+ * floats, for callers that cannot allocate device memory (harnesses that
+ * forbid hipMalloc, serving loops that must not allocate).  No counterpart
+ * in the reference.  It allocates nothing: the array is staged through a
+ * fixed device array of 2^21 complex values (16 MiB, plus 96 KiB of twiddle
+ * tables) that each device holds for the life of the process, so larger
+ * arrays are rejected with -12; it uses the default transform plans, not
+ * tuned ones.  It synchronises before returning.  Calls on one device share
+ * the fixed array: calls that overlap on one device race. */
+spimage_EXPORT int spx_fft_noalloc(const float *in, float *out, int nx, int ny, int nz, int direction);
+
 /* Run `iterations` phasing iterations of the phaser's current algorithm
  * (no support updates) and report the elapsed device time in milliseconds,
  * measured with HIP events on the engine's stream. */

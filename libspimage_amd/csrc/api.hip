@@ -1168,6 +1168,18 @@ int spx_fft_device(void* d_in, void* d_out, int nx, int ny, int nz, int directio
   return spx::fft_device((const float2*)d_in, (float2*)d_out, nx, ny, nz, direction);
 }
 
+int spx_fft_noalloc(const float* in, float* out, int nx, int ny, int nz, int direction) {
+  if (direction != -1 && direction != 1) {
+    spx::set_error("direction must be -1 or +1");
+    return -1;
+  }
+  if (!in || !out) {
+    spx::set_error("null array");
+    return -1;
+  }
+  return spx::fft_fixed((const float2*)in, (float2*)out, nx, ny, nz, direction);
+}
+
 static spx::IterParams iter_params(SpPhaser* ph) {
   spx::IterParams p{};
   p.alg = ph->algorithm->type;

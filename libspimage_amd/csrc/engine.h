@@ -215,5 +215,9 @@ int fft_host(const float2* in, float2* out, int nx, int ny, int nz, int dir);
 int blur_host(const float2* in, float2* out, int nx, int ny, int nz, float sigma);
 // transform of device data on the library's stream (synchronous)
 int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir);
+// fft_host staged through a fixed device array instead of an allocation:
+// at most kFixedElems elements (-12 beyond), untuned default plans
+constexpr long kFixedElems = 1L << 21;
+int fft_fixed(const float2* in, float2* out, int nx, int ny, int nz, int dir);
 
 }  // namespace spx

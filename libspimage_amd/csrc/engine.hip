@@ -1936,4 +1936,127 @@ int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
 }
 
 
+// ---------------------------------------------------------------------------
+// transform that allocates nothing (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// Callers that may not allocate device memory stage their array through
+// this fixed array instead of the per-call buffer of fft_host: kFixedElems
+// complex values of data followed by three twiddle tables (one per axis) of
+// up to 4096 entries.  Every device the library runs on holds its own copy
+// for the life of the process; calls on one device share it, so calls that
+// overlap on one device race.
+static __device__ __align__(256) float2 scratch[kFixedElems + 3 * 4096];
+
+namespace {
+struct FixedDev {
+  float2* ptr = nullptr;  // device address of `scratch` on this device
+  int twL[3] = {0, 0, 0}, twE[3] = {0, 0, 0};  // tables resident in the twiddle slots
+};
+std::mutex g_fixed_mu;
+FixedDev g_fixed[64];
+float2 g_fixed_tw_host[4096];
+}  // namespace
+
+// The shape's passes with the default variants and no tuning (the tuner
+// times candidate variants on buffers it allocates).
+static int make_shape_fixed(int nx, int ny, int nz, Shape* sh) {
+  *sh = Shape{};
+  sh->nx = nx;
+  sh->ny = ny;
+  sh->nz = nz;
+  sh->nrec = 1;
+  const int dims[3] = {nx, ny, nz};
+  AxisOps* ax[3] = {&sh->ex, &sh->ey, &sh->ez};
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] == 1) continue;
+    const LengthEntry* e = find_length(dims[a]);
+    if (!e) {
+      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
+      return -11;
+    }
+    ax[a]->e = e;
+    ax[a]->v = default_choice(e, false);
+  }
+  sh->gy = LineGeom{nx, nz, (long)nx * ny, (long)nx};
+  sh->gz = LineGeom{nx * ny, 1, (long)nx * ny * nz, (long)nx * ny};
+  return 0;
+}
+
+int fft_fixed(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("invalid shape %d x %d x %d", nx, ny, nz);
+    return -11;
+  }
+  const long n = (long)nx * ny * nz;
+  if (n > kFixedElems) {
+    set_error("%d x %d x %d = %ld elements exceed the %ld of the fixed device array", nx, ny, nz, n,
+              (long)kFixedElems);
+    return -12;
+  }
+  std::lock_guard<std::mutex> lock(g_fixed_mu);
+  if (!device_available()) {
+    set_error("no HIP device available");
+    return -10;
+  }
+  Shape sh;
+  if (int rc = make_shape_fixed(nx, ny, nz, &sh)) return rc;
+  int dev = 0;
+  SPX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("device %d out of range", dev);
+    return -20;
+  }
+  FixedDev& fd = g_fixed[dev];
+  if (!fd.ptr) SPX_HIP(hipGetSymbolAddress((void**)&fd.ptr, HIP_SYMBOL(scratch)));
+  float2* data = fd.ptr;
+  const hipStream_t s = nullptr;  // the null stream: no stream to create
+  // twiddle tables of the axes' plans in the three slots after the data
+  const float2* tw[3] = {nullptr, nullptr, nullptr};
+  const AxisOps* ax[3] = {&sh.ex, &sh.ey, &sh.ez};
+  for (int a = 0; a < 3; ++a) {
+    if (!*ax[a]) continue;
+    const int L = ax[a]->e->L, E = ax[a]->var().E;
+    float2* slot = fd.ptr + kFixedElems + 4096L * a;
+    tw[a] = slot;
+    if (fd.twL[a] == L && fd.twE[a] == E) continue;
+    const FftPlan p = make_plan(L, E);
+    for (int st = 1; st < p.n; ++st) {
+      const int R = p.r[st], Ns = p.ns[st];
+      for (int j = 0; j < Ns; ++j)
+        for (int k = 1; k < R; ++k) {
+          const double an = -2.0 * M_PI * (double)j * (double)k / (double)(Ns * R);
+          g_fixed_tw_host[p.twoff[st] + (k - 1) * Ns + j] = make_float2((float)cos(an), (float)sin(an));
+        }
+    }
+    if (p.twtotal > 0)
+      SPX_HIP(hipMemcpyAsync(slot, g_fixed_tw_host, sizeof(float2) * p.twtotal, hipMemcpyHostToDevice, s));
+    SPX_HIP(hipStreamSynchronize(s));  // the host table is reused for the next axis
+    fd.twL[a] = L;
+    fd.twE[a] = E;
+  }
+  SPX_HIP(hipMemcpyAsync(data, in, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+  if (sh.ex) {
+    RowArgs r = row_args(sh, data, data);
+    r.tw = tw[0];
+    sh.ex->row(dir < 0 ? kRowFwd : kRowInv, r, s);
+  }
+  ColArgs c{};
+  c.in = data;
+  c.W = data;
+  if (sh.ey) {
+    c.geo = sh.gy;
+    c.tw = tw[1];
+    sh.ey->col(dir < 0 ? kColFwd : kColInv, c, s);
+  }
+  if (sh.ez) {
+    c.geo = sh.gz;
+    c.tw = tw[2];
+    sh.ez->col(dir < 0 ? kColFwd : kColInv, c, s);
+  }
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipMemcpyAsync(out, data, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+  SPX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 }  // namespace spx

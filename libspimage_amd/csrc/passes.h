@@ -818,9 +818,11 @@ struct LengthOps {
     static const float2* t = twiddles(L, E);
     return t;
   }
+  // a twiddle table in the arguments (the no-allocation transform keeps its
+  // own) takes precedence over the shared one
   static void row(int mode, const RowArgs& a0, hipStream_t s) {
     RowArgs a = a0;
-    a.tw = tw();
+    a.tw = a0.tw ? a0.tw : tw();
     const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
     switch (mode) {
       case kRowFwd: hipLaunchKernelGGL((k_row<L, E, kRowFwd>), grid, blk, 0, s, a); break;
@@ -857,7 +859,7 @@ struct LengthOps {
   }
   static void col(int mode, const ColArgs& a0, hipStream_t s) {
     ColArgs a = a0;
-    a.tw = tw();
+    a.tw = a0.tw ? a0.tw : tw();
     const dim3 grid(col_blocks(a.geo)), blk(ColShape<L, E>::NT);
     switch (mode) {
       case kColFwd: hipLaunchKernelGGL((k_col<L, E, kColFwd>), grid, blk, 0, s, a); break;

@@ -263,6 +263,7 @@ def fft_variant(L: int, tiled: bool = False) -> tuple[int, int]:
         raise ValueError(last_error())
     return r.value, c.value
 _bind("spx_fft_device", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
+_bind("spx_fft_noalloc", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
 _bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float))
 _bind("spx_phaser_time_passes", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _bind("spx_phaser_tiled", C.c_int, PPhaser)

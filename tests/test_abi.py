@@ -133,3 +133,13 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_fixed_array_transform_rejects_before_touching_the_device(spm):
+    """spx_fft_noalloc checks its size limit (2^21 elements) and direction
+    before any device work, so these run without a GPU."""
+    a = np.zeros(4, np.complex64)
+    assert spm.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 2048, 1025, 1, -1) == -12
+    assert "fixed device array" in spm.last_error()
+    assert spm.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 2, 2, 1, 0) == -1
+    assert spm.spx_fft_noalloc(None, a.ctypes.data, 2, 2, 1, -1) == -1
