@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-spec", default="{}", help=argparse.SUPPRESS)
+    ap.add_argument("--only-headline", action="store_true",
+                    help="only the 1024^2 HIO timing (a short program for profiler runs)")
+    ap.add_argument("--only-volume", action="store_true",
+                    help="only the 256^3 reconstruction (a short program for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only launch 20 plain 64 MiB device-to-device copies (counter calibration)")
     return ap.parse_args()
@@ -162,6 +166,7 @@ def gpu_bench(args, probe_dir=None):
     gap = max(0.0, (t_f_ev + t_r_ev - t_iter) / 2)
     t_f, t_r = t_f_ev - gap, t_r_ev - gap
     tiled = sp.spx_phaser_tiled(ph) == 1
+    nt_hints = sp.spx_phaser_nt_hints(ph)
     before, sup_now = phaser_state(sp, ph)
     save_probe(probe_dir, "hio_1024", "hio", 0.9, p, sup_now, before,
                gpu_one_step(sp, "hio", 0.9, p.amplitudes, p.mask, sup_now, before))
@@ -197,8 +202,11 @@ def gpu_bench(args, probe_dir=None):
         "timing": f"median of {SAMPLES} samples of {n_sample} iterations (HIP events on the engine stream)",
         "samples_us_per_iteration": [t * 1e6 for t in samples],
         "fourier_space_layout": "4-column tiled" if tiled else "row-major",
+        "nt_hints": nt_hints,
     }
 
+    if args.only_headline:
+        return t_iter, roofline, iteration_roofline, None
     # the 1024^2 reconstruction: RAAR beta 0.87, 300 iterations, area
     # shrinkwrap every 20 (blur 3.0 -> 0.7 px, 8 % of the pixels)
     bmap, amap = sp.smap([(0, 3.0), (300, 0.7)]), sp.smap([(0, 0.08)])
@@ -557,15 +565,16 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
     iterations) through the continuous-batching server, in real time and
     with every request queued at once (capacity), and through the plain
     reference-style loop (one SpPhaser per request, in arrival order)."""
-    from concurrent.futures import ProcessPoolExecutor
-    from multiprocessing import get_context
+    from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     from libspimage_amd import spimage as sp
     from libspimage_amd import serve, workloads
     sizes = [n for n in workloads.smooth_sizes(128, 1024) if sp.spx_supported_length(n)]
     trace = workloads.serving_trace(rate, seconds, seed, sizes_2d=sizes)
     shapes = sorted({s for _, s, _, _, _ in trace})
-    with ProcessPoolExecutor(workers, mp_context=get_context("spawn")) as ex:
+    # threads (numpy's transforms release the interpreter lock): no child
+    # processes to outlive the run
+    with ThreadPoolExecutor(workers) as ex:
         pats = dict(zip(shapes, ex.map(_serving_pattern, [(s, seed + i) for i, s in enumerate(shapes)])))
 
     def requests(realtime=True, keep=None):
@@ -873,10 +882,15 @@ def main():
         raise SystemExit(2)
     import shutil
     import tempfile
+    if args.only_volume:
+        print(json.dumps({"volume_256": volume_bench()}))
+        return
+    if args.only_headline:
+        args.no_batched = args.no_serving = args.no_volume = args.no_cpu_baseline = True
     probe_dir = None if args.no_cpu_baseline else tempfile.mkdtemp(prefix="spx_accuracy_")
     try:
         t_iter, roofline, it_roof, recon = gpu_bench(args, probe_dir)
-        disc = disc_bench(probe_dir)
+        disc = None if args.only_headline else disc_bench(probe_dir)
         batched = None if args.no_batched else batched_bench()
         serving = None if args.no_serving else serving_bench()
         volume = None if args.no_volume else volume_bench(probe_dir=probe_dir)

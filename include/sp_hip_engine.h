@@ -54,6 +54,13 @@ spimage_EXPORT int spx_fft_variant(int L, int tiled, int *row_elements, int *col
  * iterates, supports and every array crossing this ABI stay row-major. */
 spimage_EXPORT int spx_phaser_tiled(SpPhaser *ph);
 
+/* Which streams of the phaser's passes carry the non-temporal hint, as the
+ * tuner chose them for its shape and batch class (or SPX_NT forced them):
+ * bit 0 Fourier-space work-array loads, bit 1 its stores, bit 2 loads of
+ * the current iterate, bit 3 stores of the next one.  Negative without an
+ * engine.  The hints change timings only, never results. */
+spimage_EXPORT int spx_phaser_nt_hints(SpPhaser *ph);
+
 /* Unnormalised c2c transform of an (nz, ny, nx) x-fastest complex64 array in
  * device memory; direction -1 forward, +1 inverse; d_in == d_out allowed.
  * Runs on the library's stream and returns after completion. */

@@ -1160,6 +1160,15 @@ int spx_phaser_tiled(SpPhaser* ph) {
   return e->tiled() ? 1 : 0;
 }
 
+int spx_phaser_nt_hints(SpPhaser* ph) {
+  Engine* e = engine_of(ph);
+  if (!e) {
+    spx::set_error("phaser has no HIP engine");
+    return -1;
+  }
+  return e->nt_hints();
+}
+
 int spx_fft_device(void* d_in, void* d_out, int nx, int ny, int nz, int direction) {
   if (direction != -1 && direction != 1) {
     spx::set_error("direction must be -1 or +1");

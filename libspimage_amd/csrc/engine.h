@@ -49,13 +49,13 @@ struct IterParams {
 
 // Lengths and line geometry of one (nz, ny, nx) shape stacked nrec times.
 struct Shape {
-  int nx, ny, nz, nrec;
+  int nx = 0, ny = 0, nz = 0, nrec = 0;
   AxisOps ex, ey, ez;  // per-axis passes with the variants chosen for this shape
   LineGeom gy, gz;
-  bool t4;       // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
-  bool colw;     // 2-D row-major: the projection pass runs with the wider work-groups
-  int nt;        // non-temporal hints: W loads / stores (bits 0, 1; 2-D only), g_k loads /
-                 // g_{k+1} stores (bits 2, 3)
+  bool t4 = false;    // 2-D: W, Wf, amplitudes, margins, phased amplitudes in the 4-column tiled layout
+  bool colw = false;  // 2-D row-major: the projection pass runs with the wider work-groups
+  int nt = 0;         // non-temporal hints: W loads / stores (bits 0, 1; 2-D only), g_k loads /
+                      // g_{k+1} stores (bits 2, 3)
   LineGeom gyt;  // y lines of the tiled layout
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
@@ -91,6 +91,7 @@ class Engine {
   int upload_model(const float2* model, int rec = -1);  // into g1
   int download(const float2* dev, float2* host, int rec = -1);
   bool tiled() const { return sh.t4; }  // Fourier-space arrays in the 4-column tiled layout
+  int nt_hints() const { return sh.nt; }
   float2* g1() { return g[cur]; }
   float2* g0() { return g[cur ^ 1]; }
   float2* gp_buf() { return gp; }
@@ -169,6 +170,9 @@ class Engine {
 
   unsigned flags_version = 0;  // bumped whenever the support bits change
   uint64_t seed = 0x5EED5EEDull;
+  // an axis has no Stockham plan: single reconstructions only, iterations
+  // unfused over Bluestein transforms (engine.hip, "Transforms of any axis length")
+  bool any_mode = false;
 
  private:
   Engine() = default;
@@ -178,6 +182,10 @@ class Engine {
   int ensure_Rf(float gamma1);
   int reduce_per_rec(int parts_per_rec, int nv, double* host_out, int count = -1);
   int blur_to_abs(float sigma, bool hist);
+  // the full transform of one or all reconstructions (shp: sh or one)
+  int fft_full(const Shape& shp, const float2* in, float2* out, int dir);
+  int iterate_any(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks);
+  float2* Sany = nullptr;  // Bluestein scratch of the any-length mode
 
   Shape sh{};
   Shape one{};  // the same shape with one reconstruction (per-slot transforms)
@@ -215,6 +223,12 @@ int fft_host(const float2* in, float2* out, int nx, int ny, int nz, int dir);
 int blur_host(const float2* in, float2* out, int nx, int ny, int nz, float sigma);
 // transform of device data on the library's stream (synchronous)
 int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir);
+// The one-shot and device transforms accept any axis length up to
+// kBluesteinMaxL besides the Stockham lengths (Bluestein's method over a
+// power-of-two Stockham transform of length <= 4096); the phasing engine
+// needs Stockham lengths.
+constexpr int kBluesteinMaxL = 2048;
+bool fft_length_supported(int L);
 // fft_host staged through a fixed device array instead of an allocation:
 // at most kFixedElems elements (-12 beyond), untuned default plans
 constexpr long kFixedElems = 1L << 21;

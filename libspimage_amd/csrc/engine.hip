@@ -125,11 +125,14 @@ struct ShapeVariants {
 };
 static std::map<std::tuple<int, int, int>, ShapeVariants> shape_variants;
 
-// variant with E elements per thread (forced by SPX_FFT_E, else 8) among
-// those usable in the layout, else the first usable one
-static int default_choice(const LengthEntry* e, bool need_t4) {
+// variant with E elements per thread (forced by SPX_FFT_E=<E> for every
+// pass, or SPX_FFT_E=<row E>,<column E>; else 8) among those usable in the
+// layout, else the first usable one
+static int default_choice(const LengthEntry* e, bool need_t4, bool column = false) {
   const char* force = getenv("SPX_FFT_E");
-  const int pref = force ? atoi(force) : 8;
+  int pref = force ? atoi(force) : 8;
+  if (force && column)
+    if (const char* comma = strchr(force, ',')) pref = atoi(comma + 1);
   int first = -1;
   for (int i = 0; i < e->nvar; ++i) {
     if (need_t4 && !e->var[i].t4) continue;
@@ -145,11 +148,37 @@ static bool tuning_off() {
   return (force && atoi(force) > 0) || (notune && atoi(notune) == 1);
 }
 
+// The specialised passes (k_col_plain, k_row_plain) serve the common case:
+// the projection pass for the phase objective without amplitude margins,
+// the real-space pass for one algorithm and beta without gp output or idle
+// slots.  SPX_NO_PLAIN=1 keeps the general kernels (for A/B runs).  The tuner
+// times the passes that will run.
+static bool plain_off() {
+  static const int off = [] {
+    const char* e = getenv("SPX_NO_PLAIN");
+    return (e && atoi(e) == 1) ? 1 : 0;
+  }();
+  return off != 0;
+}
+static bool plain_projection(int mode, const ColArgs& a) {
+  return mode == kColProject && a.objective == 0 && !a.margin && !plain_off();
+}
+
+static void launch_projection(const Variant& v, const ColArgs& ca, bool wide, hipStream_t s) {
+  if (plain_projection(kColProject, ca) && v.col_project_plain(ca, wide, s)) return;
+  if (wide) v.col_project_wide(ca, s);
+  else v.col(kColProject, ca, s);
+}
+static void launch_update(const Variant& v, const UpdateArgs& u, bool tiled, hipStream_t s) {
+  if (!plain_off() && !u.gp && !u.alg_rec && !u.beta_rec && !u.rec_active && v.update_plain(u, tiled, s)) return;
+  v.update(u, false, tiled, s);
+}
+
 // Single-pass tuning of one axis outside a 2-D pair (1-D and 3-D shapes):
 // times the pass's hot kernel on zero-filled scratch of the real geometry
 // and returns the variant index.  Caller holds tune_mu.
 static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo) {
-  int choice = default_choice(e, false);
+  int choice = default_choice(e, false, col_geo != nullptr);
   if (e->nvar > 1 && !tuning_off()) {
     const long L = e->L;
     long n;
@@ -177,7 +206,7 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
           c.W = b0;
           c.geo = *col_geo;
           c.ampx = amp;
-          ms = time_launches(s, [&] { v.col(kColProject, c, s); });
+          ms = time_launches(s, [&] { launch_projection(v, c, false, s); });
         } else {
           UpdateArgs u{};
           u.W = b0;
@@ -189,7 +218,7 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
           u.alg = kER;
           u.scale = 1.f;
           u.nsize = 1.f;
-          ms = time_launches(s, [&] { v.update(u, false, false, s); });
+          ms = time_launches(s, [&] { launch_update(v, u, false, s); });
         }
         if (getenv("SPX_TUNE_LOG"))
           fprintf(stderr, "tune length %d (%s, %d lines): E=%d  %.2f us\n", (int)L, col_geo ? "column" : "row",
@@ -248,7 +277,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   auto known = shape_variants.find(dims);
   const bool need_row = known == shape_variants.end(), need_col = need_row;
   int row_choice = need_row ? default_choice(ex, false) : known->second.v[0];
-  int col_choice = need_col ? default_choice(ey, false) : known->second.v[1];
+  int col_choice = need_col ? default_choice(ey, false, true) : known->second.v[1];
   // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows it)
   const char* not4 = getenv("SPX_NO_T4");
   const char* yes4 = getenv("SPX_T4");
@@ -350,9 +379,8 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
             [&] {
               u.gcur = ga;
               u.gnew = gb;
-              if (wd) vc.col_project_wide(ca, s);
-              else vc.col(kColProject, ca, s);
-              vr.update(u, false, t4, s);
+              launch_projection(vc, ca, wd, s);
+              launch_update(vr, u, t4, s);
               std::swap(ga, gb);
             },
             20, 30);
@@ -1083,7 +1111,7 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     a.W = out;
     a.geo = sh.gz;
     a.outer_per_rec = 1;
-    sh.ez->col(mode, a, s);
+    if (!(plain_projection(mode, a) && sh.ez->col_project_plain(a, false, s))) sh.ez->col(mode, a, s);
     if (mode != kColEFourier) {
       a = base;
       a.in = out;
@@ -1097,13 +1125,15 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
     a.W = out;
     a.geo = sh.gz;
     a.outer_per_rec = 1;
-    sh.ez->col(mode, a, s);
+    if (!(plain_projection(mode, a) && sh.ez->col_project_plain(a, false, s))) sh.ez->col(mode, a, s);
   } else {
     a.in = in;
     a.W = out;
     a.geo = tiled ? sh.gyt : sh.gy;
     a.outer_per_rec = 1;
-    if (mode == kColProject && !tiled && sh.colw) sh.ey->col_project_wide(a, s);
+    const bool wide = mode == kColProject && !tiled && sh.colw;
+    if (plain_projection(mode, a) && sh.ey->col_project_plain(a, wide, s)) return;
+    if (wide) sh.ey->col_project_wide(a, s);
     else sh.ey->col(mode, a, s);
   }
 }
@@ -1167,7 +1197,7 @@ static int tune_volume_hints(const Shape& sh0) {
           enqueue_fourier(sh, kColProject, w, w, ca, s, false);
           u.gcur = ga;
           u.gnew = gb;
-          sh.ex->update(u, false, false, s);
+          launch_update(sh.ex->var(), u, false, s);
           std::swap(ga, gb);
         },
         2, 5);
@@ -1200,6 +1230,359 @@ static int last_axis_tiles_per_rec(const Shape& sh) {
 }
 
 // ---------------------------------------------------------------------------
+// Transforms of any axis length up to kBluesteinMaxL (Bluestein's chirp-z
+// method), for the Image-level and device-buffer transforms: FFTW plans any
+// length (reference src/fft.c:79,95,163,185), the Stockham passes only
+// products of 2, 3 and 5.  With w[n] = exp(i pi n^2 / L),
+//   X[k] = conj(w[k]) sum_n (x[n] conj(w[n])) w[k - n],
+// a circular convolution of length M >= 2L - 1 (a power of two, so the
+// Stockham passes transform it): zero-padded a = x conj(w), B = FFT_M(b)/M
+// of the chirp filter b (w at 0..L-1 and at M-L+1..M-1), conv = IFFT_M(FFT_M(a) B).
+// The inverse transform is the conjugate of the forward one of conj(x).
+// w and B are computed in double on the host once per length and kept.
+// ---------------------------------------------------------------------------
+namespace {
+struct Chirp {
+  int M = 0;
+  float2* w = nullptr;  // L entries
+  float2* B = nullptr;  // M entries, FFT_M(b) / M
+};
+std::map<int, Chirp> g_chirps;
+std::mutex g_chirp_mu;
+
+void host_fft_pow2(std::vector<double>& re, std::vector<double>& im, int sign) {
+  const int n = (int)re.size();
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1)
+    for (int i = 0; i < n; i += len)
+      for (int j = 0; j < len / 2; ++j) {
+        const double an = sign * 2.0 * M_PI * (double)j / (double)len;
+        const double c = cos(an), sn = sin(an);
+        const int a = i + j, b = a + len / 2;
+        const double xr = re[b] * c - im[b] * sn, xi = re[b] * sn + im[b] * c;
+        re[b] = re[a] - xr;
+        im[b] = im[a] - xi;
+        re[a] += xr;
+        im[a] += xi;
+      }
+}
+
+__global__ void k_bs_pre(const float2* __restrict__ in, float2* __restrict__ sc, long outer, int L, int M, long inner,
+                         const float2* __restrict__ w, int conj_in) {
+  const long n = outer * M * inner;
+  GRID_STRIDE(i, n) {
+    const long ii = i % inner, r = i / inner;
+    const int m = (int)(r % M);
+    const long o = r / M;
+    float2 v = make_float2(0.f, 0.f);
+    if (m < L) {
+      float2 x = in[(o * L + m) * inner + ii];
+      if (conj_in) x.y = -x.y;
+      const float2 c = w[m];
+      v = make_float2(x.x * c.x + x.y * c.y, x.y * c.x - x.x * c.y);  // x conj(w)
+    }
+    sc[i] = v;
+  }
+}
+
+__global__ void k_bs_mul(float2* sc, long outer, int M, long inner, const float2* __restrict__ B) {
+  const long n = outer * M * inner;
+  GRID_STRIDE(i, n) {
+    const int m = (int)((i / inner) % M);
+    sc[i] = cmul(sc[i], B[m]);
+  }
+}
+
+__global__ void k_bs_post(const float2* __restrict__ sc, float2* __restrict__ out, long outer, int L, int M, long inner,
+                          const float2* __restrict__ w, int conj_out) {
+  const long n = outer * L * inner;
+  GRID_STRIDE(i, n) {
+    const long ii = i % inner, r = i / inner;
+    const int k = (int)(r % L);
+    const long o = r / L;
+    const float2 c = sc[(o * M + k) * inner + ii], wk = w[k];
+    float2 x = make_float2(c.x * wk.x + c.y * wk.y, c.y * wk.x - c.x * wk.y);  // conj(w) c
+    if (conj_out) x.y = -x.y;
+    out[i] = x;
+  }
+}
+
+// the Gaussian transfer function of kColBlur (src/image_filter.c:80-91) as
+// an element-wise step, for shapes transformed by fft_any
+__global__ void k_blur_transfer(float2* d, int nx, int ny, int nz, float sigma) {
+  const long n = (long)nx * ny * nz;
+  const double NN = (double)((float)n);
+  const double k0 = -2.0 * (3.14159265358979323846 * 3.14159265358979323846);
+  const double s2 = (double)sigma * (double)sigma;
+  GRID_STRIDE(i, n) {
+    const int ix = (int)(i % nx);
+    const long rest = i / nx;
+    const int iy = (int)(rest % ny), iz = (int)(rest / ny);
+    const float dx = (nx - ix < ix) ? (float)(nx - ix) : (float)ix;
+    const float dy = (ny - iy < iy) ? (float)(ny - iy) : (float)iy;
+    const float dz = (nz - iz < iz) ? (float)(nz - iz) : (float)iz;
+    const float dist = (float)sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+    const float r = dist / (float)nx;
+    const float h = (float)(exp(k0 * ((double)r * (double)r) * s2) / NN);
+    d[i] = make_float2(d[i].x * h, d[i].y * h);
+  }
+}
+
+// --- Engines of shapes with an axis outside the Stockham lengths run their
+// --- iterations unfused: full transforms (fft_any) and these element-wise
+// --- steps, with the per-pixel arithmetic of the fused passes.
+
+// the projection pass's work on a full transform (src/phasing.c:1061-1109)
+__global__ void k_project_any(float2* d, ColArgs a, long n) {
+  const bool plain = a.objective == 0 && !a.margin;
+  GRID_STRIDE(i, n) {
+    float2 G = d[i];
+    if (plain) {
+      G = project_phase(G, a.ampx[i], a.centro != 0);
+    } else {
+      if (a.centro) G.y = 0.f;
+      if (a.objective == 0) {
+        G = project_pixel(G, a.ampx[i], a, i);
+      } else {
+        const float2 P = a.phased_amp[i];
+        if (__float_as_uint(P.x) != kUnmeasuredBits) G = P;
+      }
+    }
+    d[i] = G;
+  }
+}
+
+// the real-space update on the inverse transform P (and Pf, the difference
+// map's transform of f1), unscaled (src/phasing.c:1111-1258)
+__global__ void k_update_any(const float2* __restrict__ P, const float2* __restrict__ Pf, UpdateArgs a, long n, int dm) {
+  const bool er = !(a.alg == kHIO || a.alg == kRAAR);
+  const float beta = a.beta;
+  const float cp = a.alg == kHIO ? -beta : 1.f - 2.f * beta;
+  const float cg = a.alg == kHIO ? 1.f : beta;
+  const SupportConstraints sc(a.constraints);
+  GRID_STRIDE(i, n) {
+    const float2 g0 = a.gcur[i];
+    const bool in = a.flags[i] & kInsideSupport;
+    const float2 Pv = P[i];
+    float2 g1;
+    if (!dm) {
+      const float2 gp = make_float2(Pv.x * a.scale, Pv.y * a.scale);
+      if (a.gp) a.gp[i] = gp;
+      float2 o = make_float2(fmaf(cp, gp.x, cg * g0.x), fmaf(cp, gp.y, cg * g0.y));
+      o = er ? make_float2(0.f, 0.f) : o;
+      const float2 gi = sc.apply(gp);
+      g1 = make_float2(in ? gi.x : o.x, in ? gi.y : o.y);
+    } else {
+      if (a.gp) a.gp[i] = Pv;
+      const float2 Pfv = Pf[i];
+      const float2 gin = make_float2(g0.x + beta * ((1.f + a.gamma2) * Pv.x / a.nsize - a.gamma2 * g0.x),
+                                     g0.y + beta * ((1.f + a.gamma2) * Pv.y / a.nsize - a.gamma2 * g0.y));
+      g1 = in ? gin : g0;
+      g1.x -= beta * Pfv.x / a.nsize;
+      g1.y -= beta * Pfv.y / a.nsize;
+      const float2 gc = sc.apply(g1);
+      g1 = in ? gc : g1;
+    }
+    a.gnew[i] = g1;
+  }
+}
+
+// E_F sums over a full transform (src/phasing.c:879-888), as kColEFourier
+__global__ __launch_bounds__(kRedThreads) void k_efourier_any(const float2* F, const float* ampx, long n,
+                                                              double* partials) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  REC_STRIDE(i, n) {
+    const float af = cabsf_ref(F[i]);
+    const float ab = ampx[i];
+    if (__float_as_uint(ab) != kUnmeasuredBits) {
+      const double aa = fabs((double)ab);
+      const double d = (double)af - aa;
+      acc[0] += d * d;
+      acc[1] += aa * aa;
+      acc[2] += aa;
+      acc[3] += (double)af;
+    } else {
+      acc[1] += (double)af * (double)af;
+    }
+  }
+  write_partials<4>(acc, partials);
+}
+
+// |S| into absb with the level-1 histogram (slice = block % 8) or the
+// running maximum per slice, as kRowBlurOut
+__global__ __launch_bounds__(256) void k_blur_abs_any(const float2* S, float* absb, unsigned* hist, unsigned* ctl, long n) {
+  float mx = 0.f;
+  const int slice = blockIdx.x & (kHistSlices - 1);
+  GRID_STRIDE(i, n) {
+    const float m = cabsf_ref(S[i]);
+    absb[i] = m;
+    mx = fmaxf(mx, m);
+    if (hist) atomicAdd(&hist[slice * 2048 + (__float_as_uint(m) >> 21)], 1u);
+  }
+  if (ctl) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&ctl[kCtlMax + slice], __float_as_uint(mx));
+  }
+}
+
+__global__ void k_abs_c64(const float2* in, float2* out, long n) {
+  GRID_STRIDE(i, n) out[i] = make_float2(cabsf_ref(in[i]), 0.f);
+}
+}  // namespace
+
+static int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+bool fft_length_supported(int L) {
+  return L >= 1 && (L == 1 || find_length(L) != nullptr || (L <= kBluesteinMaxL && find_length(next_pow2(2 * L - 1))));
+}
+
+static const Chirp* chirp_for(int L) {
+  std::lock_guard<std::mutex> lock(g_chirp_mu);
+  auto it = g_chirps.find(L);
+  if (it != g_chirps.end()) return &it->second;
+  Chirp c;
+  c.M = next_pow2(2 * L - 1);
+  std::vector<float2> w(L);
+  std::vector<double> br(c.M, 0.0), bi(c.M, 0.0);
+  for (int k = 0; k < L; ++k) {
+    const long q = ((long)k * k) % (2L * L);  // exp(i pi k^2 / L) has period 2L in k^2
+    const double an = M_PI * (double)q / (double)L;
+    w[k] = make_float2((float)cos(an), (float)sin(an));
+    br[k] = cos(an);
+    bi[k] = sin(an);
+    if (k) {
+      br[c.M - k] = cos(an);
+      bi[c.M - k] = sin(an);
+    }
+  }
+  host_fft_pow2(br, bi, -1);
+  std::vector<float2> B(c.M);
+  for (int m = 0; m < c.M; ++m) B[m] = make_float2((float)(br[m] / c.M), (float)(bi[m] / c.M));
+  if (hipMalloc(&c.w, sizeof(float2) * L) != hipSuccess || hipMalloc(&c.B, sizeof(float2) * c.M) != hipSuccess ||
+      hipMemcpy(c.w, w.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c.B, B.data(), sizeof(float2) * c.M, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("could not set up the Bluestein tables of length %d", L);
+    return nullptr;
+  }
+  return &(g_chirps[L] = c);
+}
+
+// Scratch elements fft_any needs for (nx, ny, nz): the largest padded axis.
+static long fft_any_scratch(int nx, int ny, int nz) {
+  const int dims[3] = {nx, ny, nz};
+  const long n = (long)nx * ny * nz;
+  long need = 0;
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] > 1 && !find_length(dims[a])) need = std::max(need, n / dims[a] * next_pow2(2 * dims[a] - 1));
+  return need;
+}
+
+// The unnormalised transform of every axis: Stockham passes where the
+// length has a plan (default variants), Bluestein's method elsewhere.
+static int enqueue_fft_any(const float2* in, float2* out, int nx, int ny, int nz, int dir, float2* sc, hipStream_t s) {
+  const int dims[3] = {nx, ny, nz};
+  const float2* src = in;
+  for (int a = 0; a < 3; ++a) {
+    const int L = dims[a];
+    if (L == 1) continue;
+    long inner = 1, outer = 1;
+    for (int b = 0; b < a; ++b) inner *= dims[b];
+    for (int b = a + 1; b < 3; ++b) outer *= dims[b];
+    const LengthEntry* e = find_length(L);
+    if (e) {
+      const Variant& v = e->var[default_choice(e, false)];
+      if (a == 0) {
+        RowArgs r{};
+        r.in = src;
+        r.out = out;
+        r.nlines = (int)outer;
+        r.lines_per_rec = (int)outer;
+        r.L = L;
+        v.row(dir < 0 ? kRowFwd : kRowInv, r, s);
+      } else {
+        ColArgs c{};
+        c.in = src;
+        c.W = out;
+        c.geo = LineGeom{(int)inner, (int)outer, (long)L * inner, inner};
+        v.col(dir < 0 ? kColFwd : kColInv, c, s);
+      }
+    } else {
+      const Chirp* ch = chirp_for(L);
+      if (!ch) return -20;
+      const int M = ch->M;
+      const LengthEntry* em = find_length(M);
+      const Variant& v = em->var[default_choice(em, false)];
+      const long np = outer * M * inner;
+      hipLaunchKernelGGL(k_bs_pre, ew_grid(np), dim3(256), 0, s, src, sc, outer, L, M, inner, ch->w, dir > 0 ? 1 : 0);
+      if (a == 0) {
+        RowArgs r{};
+        r.in = sc;
+        r.out = sc;
+        r.nlines = (int)outer;
+        r.lines_per_rec = (int)outer;
+        r.L = M;
+        v.row(kRowFwd, r, s);
+        hipLaunchKernelGGL(k_bs_mul, ew_grid(np), dim3(256), 0, s, sc, outer, M, inner, ch->B);
+        v.row(kRowInv, r, s);
+      } else {
+        ColArgs c{};
+        c.in = sc;
+        c.W = sc;
+        c.geo = LineGeom{(int)inner, (int)outer, (long)M * inner, inner};
+        v.col(kColFwd, c, s);
+        hipLaunchKernelGGL(k_bs_mul, ew_grid(np), dim3(256), 0, s, sc, outer, M, inner, ch->B);
+        v.col(kColInv, c, s);
+      }
+      hipLaunchKernelGGL(k_bs_post, ew_grid(outer * L * inner), dim3(256), 0, s, sc, out, outer, L, M, inner, ch->w,
+                         dir > 0 ? 1 : 0);
+    }
+    src = out;
+  }
+  if (src == in && in != out)
+    SPX_HIP(hipMemcpyAsync(out, in, sizeof(float2) * (size_t)nx * ny * nz, hipMemcpyDeviceToDevice, s));
+  return hipGetLastError() == hipSuccess ? 0 : -20;
+}
+
+// The shape's axes all have Stockham plans (the engine's own path), or not
+// but each is within fft_any's reach (1: use fft_any); -11 otherwise.
+static int any_path(int nx, int ny, int nz) {
+  const int dims[3] = {nx, ny, nz};
+  bool native = true;
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] < 1) {
+      set_error("invalid shape %d x %d x %d", nx, ny, nz);
+      return -11;
+    }
+    if (dims[a] > 1 && !find_length(dims[a])) {
+      native = false;
+      if (!fft_length_supported(dims[a])) {
+        set_error("axis length %d is not supported (products of 2, 3 and 5 up to 4096, any other length up to %d)",
+                  dims[a], kBluesteinMaxL);
+        return -11;
+      }
+    }
+  }
+  if ((long)nx * ny * nz >= (1L << 31)) {
+    set_error("array of %ld elements exceeds the int range of the phasing API", (long)nx * ny * nz);
+    return -11;
+  }
+  return native ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
 Engine* Engine::create(int nx, int ny, int nz, int nrec) {
@@ -1207,13 +1590,29 @@ Engine* Engine::create(int nx, int ny, int nz, int nrec) {
     set_error("no HIP device available");
     return nullptr;
   }
-  Shape sh, one;
-  if (make_shape(nx, ny, nz, nrec, &sh) || make_shape(nx, ny, nz, 1, &one)) return nullptr;
-  if (!sh.ex || (!sh.ey && !sh.ez)) {
+  const int path = any_path(nx, ny, nz);
+  if (path < 0) return nullptr;
+  if (nx < 2 || (ny < 2 && nz < 2)) {
     set_error("the HIP engine phases 2-D and 3-D arrays with nx > 1 (got %d x %d x %d)", nx, ny, nz);
     return nullptr;
   }
+  Shape sh, one;
+  if (path == 1) {
+    // an axis without a Stockham plan: one reconstruction, unfused iterations
+    if (nrec != 1) {
+      set_error("batches need axis lengths that are products of 2, 3 and 5 up to 4096 (got %d x %d x %d)", nx, ny,
+                nz);
+      return nullptr;
+    }
+    sh.nx = one.nx = nx;
+    sh.ny = one.ny = ny;
+    sh.nz = one.nz = nz;
+    sh.nrec = one.nrec = 1;
+  } else if (make_shape(nx, ny, nz, nrec, &sh) || make_shape(nx, ny, nz, 1, &one)) {
+    return nullptr;
+  }
   Engine* e = new Engine();
+  e->any_mode = path == 1;
   e->sh = sh;
   e->one = one;
   e->nx = nx;
@@ -1242,8 +1641,9 @@ int Engine::alloc_all() {
   SPX_HIP(hipMalloc(&ampx, f));
   SPX_HIP(hipMalloc(&absb, f));
   SPX_HIP(hipMalloc(&flags, (size_t)NT));
-  int parts = last_axis_tiles_per_rec(sh);
+  int parts = any_mode ? rec_blocks(N) : last_axis_tiles_per_rec(sh);
   if (parts < rec_blocks(N)) parts = rec_blocks(N);
+  if (any_mode) SPX_HIP(hipMalloc(&Sany, sizeof(float2) * (size_t)std::max(1L, fft_any_scratch(nx, ny, nz))));
   SPX_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)parts * nrec));
   SPX_HIP(hipMalloc(&red, sizeof(double) * 4 * (size_t)nrec));
   SPX_HIP(hipMalloc(&hist, sizeof(unsigned) * (size_t)kHistStride * nrec));
@@ -1264,7 +1664,7 @@ Engine::~Engine() {
   if (s) (void)hipStreamSynchronize(s);
   drop_graphs();
   void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min,  amp_max,   absb,
-                  phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active};
+                  phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active, Sany};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : templates) (void)hipFree(e.second);
@@ -1391,8 +1791,14 @@ int Engine::set_rec_active(const int* active) {
   return 0;
 }
 
+int Engine::fft_full(const Shape& shp, const float2* in, float2* out, int dir) {
+  if (any_mode) return enqueue_fft_any(in, out, nx, ny, nz, dir, Sany, s);
+  enqueue_fft(shp, in, out, dir, s);
+  return 0;
+}
+
 int Engine::fft_nd(const float2* in, float2* out, int dir) {
-  enqueue_fft(sh, in, out, dir, s);
+  if (int rc = fft_full(sh, in, out, dir)) return rc;
   SPX_HIP(hipGetLastError());
   return 0;
 }
@@ -1415,7 +1821,7 @@ int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd
   if (fl & 1 /* SpModelRandomPhases */ || fl & 2 /* SpModelZeroPhases */) {
     hipLaunchKernelGGL(k_init_phases, ew_grid(n), dim3(256), 0, s, absb + off, S1 + off, N, n, sd,
                        (fl & 1) ? 1 : 0);
-    enqueue_fft(shp, S1 + off, S1 + off, +1, s);
+    if (int rc = fft_full(shp, S1 + off, S1 + off, +1)) return rc;
     hipLaunchKernelGGL(k_scale_copy, ew_grid(n), dim3(256), 0, s, S1 + off, g[cur] + off, n, inv_n);
   } else if (fl & 4 /* SpModelRandomValues */) {
     // reference src/phasing.c:666-679: scale so that sum |g|^2 = sum amp^2 / N
@@ -1448,9 +1854,9 @@ int Engine::finish_model_init(int fl, int rec) {
   const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
   const Shape& shp = rec < 0 ? sh : one;
   if (fl & 256 /* SpModelMaskedOutZeroed */) {
-    enqueue_fft(shp, g[cur] + off, S1 + off, -1, s);
+    if (int rc = fft_full(shp, g[cur] + off, S1 + off, -1)) return rc;
     hipLaunchKernelGGL(k_zero_unmeasured, ew_grid(n), dim3(256), 0, s, S1 + off, flags + off, n);
-    enqueue_fft(shp, S1 + off, S1 + off, +1, s);
+    if (int rc = fft_full(shp, S1 + off, S1 + off, +1)) return rc;
     hipLaunchKernelGGL(k_scale_copy, ew_grid(n), dim3(256), 0, s, S1 + off, g[cur] + off, n,
                        (float)(1.0 / (double)N));
   }
@@ -1463,6 +1869,7 @@ int Engine::finish_model_init(int fl, int rec) {
 }
 
 int Engine::ensure_R() {
+  if (any_mode) return 0;  // unfused iterations transform the iterate in full
   if (!R_valid) {
     if (sh.t4) sh.ex->row_t4(row_args(sh, g[cur], W), s);
     else sh.ex->row(kRowFwd, row_args(sh, g[cur], W), s);
@@ -1484,6 +1891,7 @@ int Engine::ensure_R() {
 }
 
 int Engine::ensure_Rf(float gamma1) {
+  if (any_mode) return 0;  // f1 is formed and transformed in every unfused iteration
   if (!Wf) SPX_HIP(hipMalloc(&Wf, sizeof(float2) * NT));
   if (Rf_valid && Rf_version == flags_version && Rf_gamma1 == gamma1) return 0;
   if (sh.t4) {
@@ -1508,8 +1916,57 @@ int Engine::iterate(const IterParams& p, const float* betas, int n, bool want_gp
 // `marks`, when given, receives 2n+1 events: before the Fourier-space passes
 // of iteration i (2i), between them and the real-space pass (2i+1), and after
 // the last pass (2n).
+// Unfused iterations of the any-length mode: per iteration the full
+// transform of g_k, the projection, the inverse transform (and the same for
+// the difference map's f1, in W), then the update, each step element-wise
+// with the fused passes' arithmetic.  Marks as iterate_impl's.
+int Engine::iterate_any(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks) {
+  const bool dm = (p.alg == kDiffMap);
+  ColArgs ca{};
+  ca.ampx = ampx;
+  ca.amp_min = amp_min;
+  ca.amp_max = amp_max;
+  ca.phased_amp = phased;
+  ca.centro = (p.constraints & kCentrosymmetricObject) ? 1 : 0;
+  ca.margin = (p.constraints & kAmplitudeErrorMargin) ? 1 : 0;
+  ca.objective = dm ? 0 : p.objective;
+  ColArgs caf = ca;
+  caf.centro = 0;  // the difference map constrains FFT(g0) only (src/phasing.c:1232-1235)
+  UpdateArgs u{};
+  u.flags = flags;
+  u.alg = p.alg;
+  u.constraints = p.constraints;
+  u.gamma1 = p.gamma1;
+  u.gamma2 = p.gamma2;
+  u.scale = (float)(1.0 / (double)N);
+  u.nsize = (float)N;
+  for (int i = 0; i < n; ++i) {
+    if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
+    if (int rc = fft_full(sh, g[cur], S1, -1)) return rc;
+    hipLaunchKernelGGL(k_project_any, ew_grid(N), dim3(256), 0, s, S1, ca, N);
+    if (int rc = fft_full(sh, S1, S1, +1)) return rc;
+    if (dm) {
+      hipLaunchKernelGGL(k_dm_f1, ew_grid(N), dim3(256), 0, s, g[cur], flags, p.gamma1, W, N);
+      if (int rc = fft_full(sh, W, W, -1)) return rc;
+      hipLaunchKernelGGL(k_project_any, ew_grid(N), dim3(256), 0, s, W, caf, N);
+      if (int rc = fft_full(sh, W, W, +1)) return rc;
+    }
+    if (marks) SPX_HIP(hipEventRecord(marks[2 * i + 1], s));
+    u.gcur = g[cur];
+    u.gnew = g[cur ^ 1];
+    u.gp = (want_gp && i == n - 1) ? gp : nullptr;
+    u.beta = betas ? betas[i] : 0.f;
+    hipLaunchKernelGGL(k_update_any, ew_grid(N), dim3(256), 0, s, S1, W, u, N, dm ? 1 : 0);
+    cur ^= 1;
+  }
+  if (marks) SPX_HIP(hipEventRecord(marks[2 * n], s));
+  SPX_HIP(hipGetLastError());
+  return 0;
+}
+
 int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks) {
   if (n <= 0) return 0;
+  if (any_mode) return iterate_any(p, betas, n, want_gp, marks);
   const bool dm = (p.alg == kDiffMap);
   if (int r = ensure_R()) return r;
   if (dm)
@@ -1551,7 +2008,9 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
     u.gnew = g[cur ^ 1];
     u.gp = (want_gp && i == n - 1) ? gp : nullptr;
     u.beta = betas ? betas[i] : 0.f;
-    sh.ex->update(u, dm, sh.t4, s);
+    // the specialised pass: one algorithm and beta for all, no gp, no idle slots
+    const bool plain = !dm && !u.gp && !u.alg_rec && !u.beta_rec && !u.rec_active && !plain_off();
+    if (!(plain && sh.ex->update_plain(u, sh.t4, s))) sh.ex->update(u, dm, sh.t4, s);
     cur ^= 1;
   }
   if (marks) SPX_HIP(hipEventRecord(marks[2 * n], s));
@@ -1575,6 +2034,10 @@ void Engine::drop_graphs() {
 }
 
 int Engine::iterate_graph(const IterParams& p, float beta, int n) {
+  if (any_mode) {
+    std::vector<float> b((size_t)std::max(n, 1), beta);
+    return iterate_impl(p, b.data(), n, false, nullptr);
+  }
   if (n < 0) return 0;  // n == 0 captures without running
   if (int r = ensure_R()) return r;
   if (p.alg == kDiffMap)
@@ -1662,6 +2125,14 @@ int Engine::time_passes(const IterParams& p, const float* betas, int n, float* m
 }
 
 int Engine::efourier(double* out, float2* fout) {
+  if (any_mode) {
+    const int bpr = rec_blocks(N);
+    if (int rc = fft_full(sh, g[cur], S1, -1)) return rc;
+    if (fout) SPX_HIP(hipMemcpyAsync(fout, S1, sizeof(float2) * (size_t)N, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_efourier_any, dim3(bpr, 1), dim3(kRedThreads), 0, s, S1, ampx, N, partials);
+    SPX_HIP(hipGetLastError());
+    return reduce_per_rec(bpr, 4, out);
+  }
   if (int r = ensure_R()) return r;
   ColArgs ca{};
   ca.ampx = ampx;
@@ -1676,6 +2147,16 @@ int Engine::efourier(double* out, float2* fout) {
 int Engine::efourier_enqueue() {
   if (int r = ensure_R()) return r;
   if (!red_host) SPX_HIP(hipHostMalloc((void**)&red_host, sizeof(double) * 4 * (size_t)nrec, hipHostMallocDefault));
+  if (any_mode) {
+    const int bpr = rec_blocks(N);
+    if (int rc = fft_full(sh, g[cur], S1, -1)) return rc;
+    hipLaunchKernelGGL(k_efourier_any, dim3(bpr, 1), dim3(kRedThreads), 0, s, S1, ampx, N, partials);
+    hipLaunchKernelGGL(k_reduce_rec, dim3(1), dim3(256), 0, s, partials, bpr, 4, red);
+    SPX_HIP(hipGetLastError());
+    SPX_HIP(hipMemcpyAsync(red_host, red, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    red_host_pending = true;
+    return 0;
+  }
   ColArgs ca{};
   ca.ampx = ampx;
   ca.partials = partials;
@@ -1732,6 +2213,16 @@ int Engine::model_change(float2* out) {
 // |blur(|g1|, sigma)| into absb, with the level-1 histograms and the running
 // maxima (reference src/support_update.c:138-141,169-172)
 int Engine::blur_to_abs(float sigma, bool with_hist) {
+  if (any_mode) {
+    hipLaunchKernelGGL(k_abs_c64, ew_grid(N), dim3(256), 0, s, g[cur], S1, N);
+    if (int rc = fft_full(sh, S1, S1, -1)) return rc;
+    hipLaunchKernelGGL(k_blur_transfer, ew_grid(N), dim3(256), 0, s, S1, nx, ny, nz, sigma);
+    if (int rc = fft_full(sh, S1, S1, +1)) return rc;
+    hipLaunchKernelGGL(k_blur_abs_any, ew_grid(N), dim3(256), 0, s, S1, absb, with_hist ? hist : nullptr,
+                       with_hist ? nullptr : ctl, N);
+    SPX_HIP(hipGetLastError());
+    return 0;
+  }
   sh.ex->row(kRowAbsFwd, row_args(sh, g[cur], S1), s);
   ColArgs cb{};
   cb.nx = nx;
@@ -1840,6 +2331,16 @@ int Engine::support_template(unsigned long long key, const float* tmpl_abs, floa
 int Engine::patterson_support(const float* amp_raw, float value) {
   SPX_HIP(hipMemcpyAsync(absb, amp_raw, sizeof(float) * NT, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_amp2, ew_grid(NT), dim3(256), 0, s, absb, S1, NT);
+  if (any_mode) {
+    if (int rc = fft_full(sh, S1, S1, +1)) return rc;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, ctl, nrec, 0u);
+    hipLaunchKernelGGL(k_blur_abs_any, ew_grid(N), dim3(256), 0, s, S1, absb, nullptr, ctl, N);
+    hipLaunchKernelGGL(k_mask, dim3(rec_blocks(N), nrec), dim3(256), 0, s, absb, flags, N, ctl, 1, value);
+    SPX_HIP(hipGetLastError());
+    SPX_HIP(hipStreamSynchronize(s));
+    ++flags_version;
+    return 0;
+  }
   ColArgs c{};
   if (sh.ey) {
     c.in = S1;
@@ -1876,20 +2377,36 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
     set_error("no HIP device available");
     return -10;
   }
-  Shape sh;
-  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
-  if (is_blur && (!sh.ex || (!sh.ey && !sh.ez))) {
+  const int path = any_path(nx, ny, nz);
+  if (path < 0) return path;
+  if (is_blur && (nx < 2 || (ny < 2 && nz < 2))) {
     set_error("blur needs a 2-D or 3-D array with nx > 1");
     return -11;
   }
+  Shape sh;
+  if (path == 0)
+    if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
   static hipStream_t s = nullptr;
   if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   const size_t bytes = sizeof(float2) * (size_t)nx * ny * nz;
   float2* d = nullptr;
+  float2* sc = nullptr;
   SPX_HIP(hipMalloc(&d, bytes));
+  if (path == 1 && hipMalloc(&sc, sizeof(float2) * (size_t)fft_any_scratch(nx, ny, nz)) != hipSuccess) {
+    (void)hipFree(d);
+    set_error("out of device memory for the transform scratch");
+    return -20;
+  }
   int rc = 0;
   if (hipMemcpyAsync(d, in, bytes, hipMemcpyHostToDevice, s) != hipSuccess) rc = -20;
-  if (!rc) {
+  if (!rc && path == 1) {
+    // any length: transform, the transfer function element-wise, inverse
+    rc = enqueue_fft_any(d, d, nx, ny, nz, is_blur ? -1 : dir, sc, s);
+    if (!rc && is_blur) {
+      hipLaunchKernelGGL(k_blur_transfer, ew_grid((long)nx * ny * nz), dim3(256), 0, s, d, nx, ny, nz, sigma);
+      rc = enqueue_fft_any(d, d, nx, ny, nz, +1, sc, s);
+    }
+  } else if (!rc) {
     if (!is_blur) {
       enqueue_fft(sh, d, d, dir, s);
     } else {
@@ -1907,6 +2424,7 @@ static int oneshot(const float2* in, float2* out, int nx, int ny, int nz, int di
   if (!rc && hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = -20;
   if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = -20;
   (void)hipFree(d);
+  if (sc) (void)hipFree(sc);
   if (rc == -20) set_error("HIP error in one-shot transform");
   return rc;
 }
@@ -1925,10 +2443,21 @@ int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir) {
     set_error("no HIP device available");
     return -10;
   }
-  Shape sh;
-  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
+  const int path = any_path(nx, ny, nz);
+  if (path < 0) return path;
   static hipStream_t s = nullptr;
   if (!s) SPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  if (path == 1) {
+    float2* sc = nullptr;
+    SPX_HIP(hipMalloc(&sc, sizeof(float2) * (size_t)fft_any_scratch(nx, ny, nz)));
+    int rc = enqueue_fft_any(in, out, nx, ny, nz, dir, sc, s);
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = -20;
+    (void)hipFree(sc);
+    if (rc) set_error("HIP error in the transform");
+    return rc;
+  }
+  Shape sh;
+  if (int rc = make_shape(nx, ny, nz, 1, &sh)) return rc;
   enqueue_fft(sh, in, out, dir, s);
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));

@@ -81,10 +81,10 @@ struct LineGeom {
   // line element i of line (outer, c) lives at
   //   outer*outer_stride + (c >> tshift)*tstride + (c & (2^tshift - 1)) + i*axis_stride
   // natural layout: tshift large (one tile); 4-column tiled layout: tshift 2
-  int ncontig;       // lines of one outer index (x extent for column passes)
-  int nouter;        // number of outer indices
-  long outer_stride;
-  long axis_stride;
+  int ncontig = 0;       // lines of one outer index (x extent for column passes)
+  int nouter = 0;        // number of outer indices
+  long outer_stride = 0;
+  long axis_stride = 0;
   int tshift = 30;
   long tstride = 0;
 };
@@ -218,6 +218,23 @@ __device__ __forceinline__ float2 project_pixel(float2 g, float ampbits_f, const
   }
   if (isfinite(m)) return make_float2(g.x * m, g.y * m);
   return make_float2(amp, 0.f);
+}
+
+// The modulus projection of project_pixel without margins, branch-free,
+// after the centrosymmetric constraint (src/phasing.c:1061-1100).
+// m = amp / |G| is computed as amp * rsqrt(|G|^2): 1-2 ulp from the
+// correctly rounded quotient, a third of its instructions.  A denormal
+// |G|^2 is scaled by 2^64 first (exactly), as the reciprocal square root
+// flushes denormal inputs; |G|^2 = 0 still gives an infinite m.
+__device__ __forceinline__ float2 project_phase(float2 G, float amp, bool centro) {
+  G.y = centro ? 0.f : G.y;
+  const float sq = G.y * G.y + G.x * G.x;
+  const bool tiny = sq < 1.17549435e-38f;
+  const float m = amp * rsqrtf(tiny ? sq * 0x1p64f : sq) * (tiny ? 0x1p32f : 1.f);
+  const bool fin = isfinite(m);
+  const float2 P = make_float2(fin ? G.x * m : amp, fin ? G.y * m : 0.f);
+  const bool unm = __float_as_uint(amp) == kUnmeasuredBits;
+  return make_float2(unm ? G.x : P.x, unm ? G.y : P.y);
 }
 
 // Real-space support constraints (src/phasing.c:1026-1059), inside support only.
@@ -585,6 +602,96 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
   }
 }
 
+// The real-space pass for the common case, specialised: ER / HIO / RAAR with
+// one algorithm and beta for every line, no model-before-projection output,
+// no idle reconstructions.  W's cache hint is a template argument (NTW 0 or
+// 3: loads and stores alike); the iterates' hints (nt bits 2, 3) stay
+// arguments.  The same arithmetic as k_row_update, without its optional
+// paths: kept apart, those cost this pass about 0.4 us at 1024^2.
+template <int L, int E, bool T4, int NTW>
+__global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs a) {
+  using PL = PlanOf<L, E>;
+  using RG = RowGeo<L, E, T4>;
+  constexpr int M = RG::M;
+  constexpr int NR = PL::NREG;
+  using View = LdsView<L, M, false>;
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
+  const int tid = threadIdx.x;
+  const int g = RG::line_of(tid), t = RG::pos_of(tid);
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int line = blk * M + g;
+  const bool act = line < a.nlines;
+  const View va{shA, g}, vb{shB, g};
+  const int lc = act ? line : a.nlines - 1;
+  const long base = (long)lc * L;
+  long wbase = base;
+  if constexpr (T4) {
+    const int rec = lc / a.lines_per_rec, y = lc - rec * a.lines_per_rec;
+    wbase = (long)rec * L * a.lines_per_rec + 4L * y;
+  }
+  auto wpos = [&](int x) -> long {
+    if constexpr (T4) return t4_offset(x, a.lines_per_rec);
+    else return x;
+  };
+  float2 v[NR], g0v[NR];
+  uint8_t fl[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const float2* p = &a.W[wbase + wpos(in_pos<L, E>(t, q))];
+    v[q] = NTW ? ld_nt(p) : *p;
+  }
+  if (a.nt & 4) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) g0v[q] = ld_nt(&a.gcur[base + out_pos<L, E>(t, q)]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) g0v[q] = a.gcur[base + out_pos<L, E>(t, q)];
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) fl[q] = a.flags[base + out_pos<L, E>(t, q)];
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  fft_line<L, E, +1>(v, va, vb, t, twr);
+  // src/phasing.c:1128-1136 (ER), 1160-1168 (HIO), 1194-1215 (RAAR)
+  const int alg = a.alg;
+  const float beta = a.beta;
+  const bool er = !(alg == kHIO || alg == kRAAR);
+  const float cp = alg == kHIO ? -beta : 1.f - 2.f * beta;
+  const float cg = alg == kHIO ? 1.f : beta;
+  const SupportConstraints sc(a.constraints);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const float2 gp = make_float2(v[q].x * a.scale, v[q].y * a.scale);
+    const float2 g0 = g0v[q];
+    float2 o = make_float2(fmaf(cp, gp.x, cg * g0.x), fmaf(cp, gp.y, cg * g0.y));
+    o = er ? make_float2(0.f, 0.f) : o;
+    const float2 gi = sc.apply(gp);
+    const bool in = fl[q] & kInsideSupport;
+    v[q] = make_float2(in ? gi.x : o.x, in ? gi.y : o.y);
+  }
+  if (act) {
+    if (a.nt & 8) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) st_nt(&a.gnew[base + out_pos<L, E>(t, q)], v[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) a.gnew[base + out_pos<L, E>(t, q)] = v[q];
+    }
+  }
+  out_to_in<L, E>(v, va, t);
+  __syncthreads();
+  fft_line<L, E, -1>(v, va, vb, t, twr);
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      float2* p = &a.W[wbase + wpos(out_pos<L, E>(t, q))];
+      if constexpr (NTW != 0) st_nt(p, v[q]);
+      else *p = v[q];
+    }
+  }
+}
+
 // Forward x-transform of row-major `in` into the 4-column tiled `out`
 // (in != out): the row-transformed iterate W of a 2-D shape.
 template <int L, int E>
@@ -700,22 +807,7 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
       // project_pixel without margins (src/phasing.c:1070-1100)
       const bool centro = a.centro != 0;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        float2 G = v[q];
-        G.y = centro ? 0.f : G.y;  // src/phasing.c:1061-1068
-        const float amp = ampv[q];
-        // m = amp / |G| as amp * rsqrt(|G|^2): 1-2 ulp from the correctly
-        // rounded quotient, a third of its instructions.  A denormal |G|^2 is
-        // scaled by 2^64 first (exactly), as the reciprocal square root
-        // flushes denormal inputs; |G|^2 = 0 still gives an infinite m.
-        const float sq = G.y * G.y + G.x * G.x;
-        const bool tiny = sq < 1.17549435e-38f;
-        const float m = amp * rsqrtf(tiny ? sq * 0x1p64f : sq) * (tiny ? 0x1p32f : 1.f);
-        const bool fin = isfinite(m);
-        const float2 P = make_float2(fin ? G.x * m : amp, fin ? G.y * m : 0.f);
-        const bool unm = __float_as_uint(amp) == kUnmeasuredBits;
-        v[q] = make_float2(unm ? G.x : P.x, unm ? G.y : P.y);
-      }
+      for (int q = 0; q < NR; ++q) v[q] = project_phase(v[q], ampv[q], centro);
     } else {
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
@@ -806,6 +898,59 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
   }
 }
 
+// The Fourier-space pass for the common case, specialised: the phase
+// objective without amplitude margins (the branch-free projection of
+// k_col), W's cache hint a template argument (NTW 0 or 3).  Idle
+// reconstructions of a batch still leave at once.
+template <int L, int E, bool WIDE, int NTW>
+__global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col_plain(ColArgs a) {
+  using PL = PlanOf<L, E>;
+  constexpr int C = ColShape<L, E, WIDE>::C;
+  constexpr int NR = PL::NREG;
+  using View = LdsView<L, C, true>;
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  const int ngrp = (a.geo.ncontig + C - 1) / C;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int outer = blk / ngrp;
+  if (a.rec_active && !a.rec_active[outer / a.outer_per_rec]) return;
+  const int x = (blk - outer * ngrp) * C + c;
+  const bool act = x < a.geo.ncontig;
+  const int xc = act ? x : a.geo.ncontig - 1;
+  const long base = (long)outer * a.geo.outer_stride + (long)(xc >> a.geo.tshift) * a.geo.tstride +
+                    (xc & ((1 << a.geo.tshift) - 1));
+  const long st = a.geo.axis_stride;
+  const View va{shA, c}, vb{shB, c};
+  float2 v[NR];
+  float ampv[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const float2* p = &a.in[base + in_pos<L, E>(t, q) * st];
+    v[q] = NTW ? ld_nt(p) : *p;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) ampv[q] = a.ampx[base + out_pos<L, E>(t, q) * st];
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  fft_line<L, E, -1>(v, va, vb, t, twr);
+  const bool centro = a.centro != 0;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = project_phase(v[q], ampv[q], centro);
+  out_to_in<L, E>(v, va, t);
+  __syncthreads();
+  fft_line<L, E, +1>(v, va, vb, t, twr);
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      float2* p = &a.W[base + out_pos<L, E>(t, q) * st];
+      if constexpr (NTW != 0) st_nt(p, v[q]);
+      else *p = v[q];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers and the per-length dispatch table.
 // ---------------------------------------------------------------------------
@@ -846,6 +991,46 @@ struct LengthOps {
     const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
     if (dm) hipLaunchKernelGGL((k_row_update<L, E, true, false>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k_row_update<L, E, false, false>), grid, blk, 0, s, a);
+  }
+  // the specialised passes; false when the arguments need the general ones
+  // (a W hint other than none or loads and stores both)
+  static bool update_plain(const UpdateArgs& a0, bool tiled, hipStream_t s) {
+    UpdateArgs a = a0;
+    a.tw = tw();
+    const int ntw = a.nt & 3;
+    if (ntw != 0 && ntw != 3) return false;
+    if (tiled) {
+      if constexpr (t4) {
+        const dim3 grid((a.nlines + 3) / 4), blk(RowGeo<L, E, true>::NT);
+        if (ntw) hipLaunchKernelGGL((k_row_plain<L, E, true, 3>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_row_plain<L, E, true, 0>), grid, blk, 0, s, a);
+        return true;
+      }
+      return false;
+    }
+    const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
+    if (ntw) hipLaunchKernelGGL((k_row_plain<L, E, false, 3>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_row_plain<L, E, false, 0>), grid, blk, 0, s, a);
+    return true;
+  }
+  template <bool WIDE>
+  static bool col_plain_launch(const ColArgs& a0, hipStream_t s) {
+    ColArgs a = a0;
+    a.tw = tw();
+    const int ntw = a.nt & 3;
+    if (ntw != 0 && ntw != 3) return false;
+    const dim3 grid(((a.geo.ncontig + ColShape<L, E, WIDE>::C - 1) / ColShape<L, E, WIDE>::C) * a.geo.nouter),
+        blk(ColShape<L, E, WIDE>::NT);
+    if (ntw) hipLaunchKernelGGL((k_col_plain<L, E, WIDE, 3>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_col_plain<L, E, WIDE, 0>), grid, blk, 0, s, a);
+    return true;
+  }
+  static bool col_project_plain(const ColArgs& a0, bool wide, hipStream_t s) {
+    if (wide) {
+      if constexpr (ColShape<L, E, true>::C > ColShape<L, E>::C) return col_plain_launch<true>(a0, s);
+      return false;
+    }
+    return col_plain_launch<false>(a0, s);
   }
   static void row_t4(const RowArgs& a0, hipStream_t s) {
     if constexpr (t4) {
@@ -896,6 +1081,8 @@ struct Variant {
   int (*col_blocks)(const LineGeom&);
   bool wide;  // has a wider projection pass (col_project_wide)
   void (*col_project_wide)(const ColArgs&, hipStream_t);
+  bool (*update_plain)(const UpdateArgs&, bool, hipStream_t);
+  bool (*col_project_plain)(const ColArgs&, bool, hipStream_t);
 };
 
 template <int L, int E>
@@ -908,7 +1095,9 @@ constexpr Variant variant() {
                  &LengthOps<L, E>::col,
                  &LengthOps<L, E>::col_blocks,
                  LengthOps<L, E>::wide,
-                 &LengthOps<L, E>::col_project_wide};
+                 &LengthOps<L, E>::col_project_wide,
+                 &LengthOps<L, E>::update_plain,
+                 &LengthOps<L, E>::col_project_plain};
 }
 
 // Dispatch table entry for one length: its compiled variants.
@@ -932,6 +1121,10 @@ struct AxisOps {
   void row_t4(const RowArgs& a, hipStream_t s) const { var().row_t4(a, s); }
   void col(int mode, const ColArgs& a, hipStream_t s) const { var().col(mode, a, s); }
   void col_project_wide(const ColArgs& a, hipStream_t s) const { var().col_project_wide(a, s); }
+  bool update_plain(const UpdateArgs& a, bool tiled, hipStream_t s) const { return var().update_plain(a, tiled, s); }
+  bool col_project_plain(const ColArgs& a, bool wide, hipStream_t s) const {
+    return var().col_project_plain(a, wide, s);
+  }
   int col_blocks(const LineGeom& g) const { return var().col_blocks(g); }
 };
 

@@ -267,6 +267,7 @@ _bind("spx_fft_noalloc", C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_
 _bind("spx_phaser_time_iterations", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float))
 _bind("spx_phaser_time_passes", C.c_int, PPhaser, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float))
 _bind("spx_phaser_tiled", C.c_int, PPhaser)
+_bind("spx_phaser_nt_hints", C.c_int, PPhaser)
 _bind("spx_phaser_device_arrays", C.c_int, PPhaser, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
       C.POINTER(C.c_void_p))
 _bind("spx_phaser_set_seed", None, PPhaser, C.c_ulonglong)

@@ -63,8 +63,61 @@ def test_fft_device_entry(sp, o64):
     finally:
         hip.hipFree(d)
     assert rel(out, o64.fft(x, -1)) < 2e-6
-    assert sp.spx_fft_device(None, None, 7 * 11, 4, 1, -1) < 0  # 77 is not 2^a 3^b 5^c
+    assert sp.spx_fft_device(None, None, 4099, 4, 1, -1) < 0  # a prime above 2048
     assert "not supported" in sp.last_error()
+
+
+# lengths without a Stockham plan go through Bluestein's method: 7-smooth,
+# primes, mixed with plan lengths, and 2047 (the largest padding, 4096)
+@pytest.mark.parametrize("shape", [(1, 97), (1, 2047), (49, 64), (448, 30), (13, 11, 17), (7, 1000), (3, 121, 8),
+                                   (30, 7, 2)])
+def test_fft_any_length_matches_oracle(sp, o64, shape):
+    rng = np.random.default_rng(sum(shape) + 7)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    img = sp.image_from_array(x)
+    f = sp.sp_image_fft(img)
+    assert bool(f), sp.last_error()
+    fw = sp.image_view(f).copy()
+    b = sp.sp_image_ifft(f)
+    back = sp.image_view(b).copy()
+    for i in (img, f, b):
+        sp.sp_image_free(i)
+    e_f, e_b = rel(fw, o64.fft(x, -1)), rel(back / x.size, x)
+    print(f"{shape}: forward {e_f:.2e} from the FP64 oracle, round trip {e_b:.2e}")
+    assert e_f < 3e-6
+    assert e_b < 5e-6
+
+
+def test_fft_any_length_device_entry(sp, o64):
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    shape = (21, 77)
+    x = (np.random.default_rng(5).standard_normal(shape) * (1 - 2j)).astype(np.complex64)
+    d, e = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d), C.c_size_t(x.nbytes)) == 0
+    assert hip.hipMalloc(C.byref(e), C.c_size_t(x.nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(d, x.ctypes.data_as(C.c_void_p), C.c_size_t(x.nbytes), 1) == 0
+        assert sp.spx_fft_device(d, e, shape[1], shape[0], 1, +1) == 0, sp.last_error()
+        out = np.empty_like(x)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), e, C.c_size_t(x.nbytes), 2) == 0
+    finally:
+        hip.hipFree(d)
+        hip.hipFree(e)
+    assert rel(out, o64.fft(x, +1)) < 3e-6
+
+
+def test_blur_any_length_matches_oracle(sp, o32):
+    shape = (49, 50)
+    rng = np.random.default_rng(9)
+    x = rng.random(shape).astype(np.complex64)
+    img = sp.image_from_array(x)
+    out = sp.sp_gaussian_blur(img, 2.0)
+    assert bool(out), sp.last_error()
+    got = sp.image_view(out).copy()
+    sp.sp_image_free(out)
+    sp.sp_image_free(img)
+    assert rel(got, o32.blur(x, 2.0)) < 3e-6
 
 
 # ---------------------------------------------------------------------------
