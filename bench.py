@@ -59,11 +59,22 @@ def parse():
     ap.add_argument("--cpu-spec", default="{}", help=argparse.SUPPRESS)
     ap.add_argument("--only-headline", action="store_true",
                     help="only the 1024^2 HIO timing (a short program for profiler runs)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only the many-reconstruction batched workload (a short program for profiler runs)")
     ap.add_argument("--only-volume", action="store_true",
                     help="only the 256^3 reconstruction (a short program for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only launch 20 plain 64 MiB device-to-device copies (counter calibration)")
     return ap.parse_args()
+
+
+def _profile_bytes(name):
+    """A committed counter summary under profiles/ (HBM bytes rocprof counted
+    per pixel or voxel of a workload's passes), or {} when absent."""
+    try:
+        return json.loads((ROOT / "profiles" / name).read_text())
+    except (OSError, ValueError):
+        return {}
 
 
 def make_hio_phaser(sp, pattern, beta=0.9, alg="hio", sup_array=None):
@@ -330,6 +341,23 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             b.iterate(min(quantum, iters - q))
     ef_conc = {n: b.efourier()[0] for n, b in bs.items()}
     t_conc = time.perf_counter() - t0
+    # device time of each class's plain HIO iteration (no support update),
+    # against the survey's 78 N bytes and against the bytes rocprof counted
+    # for that class's two passes
+    phys = _profile_bytes("batched_hbm_traffic.json")
+    device, survey_bytes = {}, 0.0
+    for n, b in bs.items():
+        nrec = len(classes[n]) * n_starts
+        b.time_iterations(5)
+        us = b.time_iterations(50) / 50 * 1e3
+        row = {"recons": nrec, "hio_us_per_iteration": us,
+               "frac_survey_bytes": 78.0 * n * n * nrec / (us * 1e-6) / PEAK_HBM}
+        bpp = phys.get(f"{n}x{n}", {}).get("bytes_per_pixel") if phys else None
+        if bpp:
+            row["physical_bytes_per_pixel"] = bpp
+            row["frac_physical_bytes"] = bpp * n * n * nrec / (us * 1e-6) / PEAK_HBM
+        device[f"{n}x{n}"] = row
+        survey_bytes += 78.0 * n * n * nrec * iters
     for b in bs.values():
         b.close()
     same = all(np.array_equal(ef_conc[n], ef_serial[n]) for n in classes)
@@ -374,7 +402,11 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             "plain_seconds_extrapolated": t_plain, "plain_recons_per_s": total / t_plain,
             "speedup": t_plain / t_conc, "speedup_serial_classes": t_plain / t_serial,
             "plain_sample": f"{plain_per_size} reconstructions per size class",
-            "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size}
+            "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size,
+            "hio_iterations_device": device,
+            "whole_run_frac_survey_bytes": survey_bytes / t_conc / PEAK_HBM,
+            "whole_run_note": "the survey's 78 N bytes of every reconstruction-iteration over the concurrent run's "
+                              "wall time, shrinkwrap updates included in the time but not in the bytes"}
 
 
 def mixed_radix_bench(sizes=(720, 750, 810, 900, 972, 1000, 768, 960, 1024), nrec=16, iters=100, seed=7):
@@ -439,6 +471,7 @@ def volume_bench(dm_its=200, er_its=100, period=20, probe_dir=None):
         return ph
 
     per_it = {}
+    phys = _profile_bytes("volume_256_hbm_traffic.json")
     for name, alg, nbytes in (("dm", dm, 211), ("er", er, 102)):
         ph = phaser(alg, None)
         ms = C.c_float()
@@ -448,6 +481,10 @@ def volume_bench(dm_its=200, er_its=100, period=20, probe_dir=None):
         t = ms.value / 1e3 / 40
         per_it[name] = {"us_per_iteration": t * 1e6, "algorithmic_bytes": nbytes * N,
                         "achieved_GBps": nbytes * N / t / 1e9, "frac": nbytes * N / t / PEAK_HBM}
+        bpv = phys.get(name, {}).get("bytes_per_voxel") if phys else None
+        if bpv:
+            per_it[name].update({"physical_bytes": bpv * N, "physical_GBps": bpv * N / t / 1e9,
+                                 "frac_physical_bytes": bpv * N / t / PEAK_HBM})
         if name == "dm" and probe_dir:
             before, sup_now = phaser_state(sp, ph)
             save_probe(probe_dir, "dm_256cubed", "dm", 0.8, p, sup_now, before,
@@ -884,6 +921,9 @@ def main():
     import tempfile
     if args.only_volume:
         print(json.dumps({"volume_256": volume_bench()}))
+        return
+    if args.only_batched:
+        print(json.dumps({"batched": batched_bench()}))
         return
     if args.only_headline:
         args.no_batched = args.no_serving = args.no_volume = args.no_cpu_baseline = True

@@ -39,8 +39,12 @@ spimage_EXPORT int spx_supported_length(int L);
  * or batch of that shape and size class would (variants per shape; layout
  * and column-pass width per class of nrec: 1, 2-3, 4-7, 8 or more), so that
  * later engines and batches start without timing runs.  A server calls it
- * for the shapes and batch sizes it expects.  0, or -11 for an unsupported
- * shape, -20 without a device. */
+ * for the shapes and batch sizes it expects.  With the environment variable
+ * SPX_TUNE_FILE=<path>, every timed choice is appended to that file as a
+ * line of text, and a process reads the file before its first choice, so
+ * that a restarted server (or a profiler run) makes the same choices
+ * without timing anything.  0, or -11 for an unsupported shape, -20 without
+ * a device. */
 spimage_EXPORT int spx_tune_shape(int nx, int ny, int nz, int nrec);
 /* Elements per thread (4, 8 or 16) of the transform variants the tuner chose
  * for L x L arrays in row passes and in column passes (tuning the shape if

@@ -268,6 +268,82 @@ struct PairChoice {
   int nt;
 };
 static std::map<std::tuple<int, int, int>, PairChoice> tune_layout;
+// iterate hints chosen per 3-D shape and batch class (tune_volume_hints)
+static std::map<std::tuple<int, int, int, int>, int> volume_hints;
+
+// Tuning records.  With SPX_TUNE_FILE=<path>, every choice the tuner times
+// is appended to the file as one line of text, and the file's lines are read
+// before the process's first choice, so a later process (a restarted server,
+// a profiler run) makes the same choices without timing anything:
+//   shape <nx> <ny> <nz> <Ex> <Ey> <Ez>        elements per thread (0: axis of length 1)
+//   pair <nx> <ny> <class> <tiled> <wide> <hints>
+//   volume <nx> <ny> <nz> <class> <hints>
+// (class: batch class 0-3, hints: the non-temporal mask).  Lines that name a
+// variant this build lacks, or that do not parse, are skipped.  Caller holds
+// tune_mu.
+static const char* tune_file() {
+  const char* p = getenv("SPX_TUNE_FILE");
+  return (p && *p) ? p : nullptr;
+}
+
+static int variant_of(int L, int E) {
+  if (L == 1) return E == 0 ? 0 : -1;
+  const LengthEntry* e = find_length(L);
+  if (!e) return -1;
+  for (int i = 0; i < e->nvar; ++i)
+    if (e->var[i].E == E) return i;
+  return -1;
+}
+
+static int elements_of(int L, int v) {
+  if (L == 1) return 0;
+  const LengthEntry* e = find_length(L);
+  return e && v >= 0 && v < e->nvar ? e->var[v].E : 0;
+}
+
+static void load_tune_file() {
+  static bool loaded = false;
+  if (loaded) return;
+  loaded = true;
+  const char* path = tune_file();
+  FILE* f = path ? fopen(path, "r") : nullptr;
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    int a[6];
+    if (sscanf(line, "shape %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) == 6) {
+      ShapeVariants sv{};
+      bool ok = a[0] >= 1 && a[1] >= 1 && a[2] >= 1;
+      for (int k = 0; ok && k < 3; ++k) {
+        sv.v[k] = variant_of(a[k], a[3 + k]);
+        ok = sv.v[k] >= 0;
+      }
+      if (ok) shape_variants[std::make_tuple(a[0], a[1], a[2])] = sv;
+    } else if (sscanf(line, "pair %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) == 6) {
+      tune_layout[std::make_tuple(a[0], a[1], a[2])] = PairChoice{a[3] != 0, a[4] != 0, a[5] & 15};
+    } else if (sscanf(line, "volume %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
+      volume_hints[std::make_tuple(a[0], a[1], a[2], a[3])] = a[4] & 15;
+    }
+  }
+  fclose(f);
+}
+
+static void record_tune_line(const char* fmt, ...) {
+  const char* path = tune_file();
+  if (!path) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(f, fmt, ap);
+  va_end(ap);
+  fclose(f);
+}
+
+static void record_shape(int nx, int ny, int nz, const ShapeVariants& sv) {
+  record_tune_line("shape %d %d %d %d %d %d\n", nx, ny, nz, elements_of(nx, sv.v[0]), elements_of(ny, sv.v[1]),
+                   elements_of(nz, sv.v[2]));
+}
 
 static int batch_class(int nrec) { return nrec <= 1 ? 0 : nrec < 4 ? 1 : nrec < 8 ? 2 : 3; }
 
@@ -291,7 +367,10 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   const char* ntenv = getenv("SPX_NT");
   auto finish = [&](PairChoice pc) {
     *nt = ntenv ? (atoi(ntenv) & 15) : pc.nt;
-    if (need_row) shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
+    if (need_row) {
+      shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
+      if (!tuning_off()) record_shape(ex->L, ny, 1, shape_variants[dims]);
+    }
     *vx = row_choice;
     *vy = col_choice;
     const bool t4 = decided(pc.t4);
@@ -411,6 +490,7 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     if (p) (void)hipFree(p);
   if (s) (void)hipStreamDestroy(s);
   tune_layout[key] = PairChoice{blay == 1, bwide, bnt};
+  record_tune_line("pair %d %d %d %d %d %d\n", nx, ny, batch_class(nrec), blay == 1 ? 1 : 0, bwide ? 1 : 0, bnt);
   if (getenv("SPX_TUNE_LOG"))
     fprintf(stderr, "tune %d x %d (x%d) chose %s%s, hints %d: row E=%d col E=%d\n", nx, ny, nr,
             blay == 1 ? "tiled" : "row-major", bwide ? " wide" : "", bnt, ex->var[br].E, ey->var[bc].E);
@@ -1018,6 +1098,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt.tshift = 2;
   sh->gyt.tstride = 4L * ny;
   std::lock_guard<std::mutex> lock(tune_mu);
+  load_tune_file();
   if (found[0] && found[1] && !found[2]) {
     sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw, &sh->nt);
     return 0;
@@ -1031,6 +1112,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
     if (found[1]) sh->ey.v = tune_length(found[1], 0, &sh->gy);
     if (found[2]) sh->ez.v = tune_length(found[2], 0, &sh->gz);
     shape_variants[key] = ShapeVariants{{sh->ex.v, sh->ey.v, sh->ez.v}};
+    if (!tuning_off()) record_shape(nx, ny, nz, shape_variants[key]);
   }
   if (found[0] && found[1] && found[2]) sh->nt = tune_volume_hints(*sh);
   return 0;
@@ -1145,8 +1227,6 @@ static void enqueue_fourier(const Shape& sh, int mode, const float2* in, float2*
 // iterates took 256^3 from 269 to 247 us per HIO iteration and 96^3 x 32
 // from 546 to 500 us, but 64^3 x 32 from 122 to 129 us.  SPX_NT=<mask>
 // forces the iterate bits.  Caller holds tune_mu.
-static std::map<std::tuple<int, int, int, int>, int> volume_hints;
-
 static int tune_volume_hints(const Shape& sh0) {
   const char* ntenv = getenv("SPX_NT");
   if (ntenv) return atoi(ntenv) & 12;
@@ -1214,6 +1294,7 @@ static int tune_volume_hints(const Shape& sh0) {
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (s) (void)hipStreamDestroy(s);
+  record_tune_line("volume %d %d %d %d %d\n", sh0.nx, sh0.ny, sh0.nz, batch_class(sh0.nrec), best_mask);
   return volume_hints[key] = best_mask;
 }
 
