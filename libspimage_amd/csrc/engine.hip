@@ -170,7 +170,7 @@ static void launch_projection(const Variant& v, const ColArgs& ca, bool wide, hi
   else v.col(kColProject, ca, s);
 }
 static void launch_update(const Variant& v, const UpdateArgs& u, bool tiled, hipStream_t s) {
-  if (!plain_off() && !u.gp && !u.alg_rec && !u.beta_rec && !u.rec_active && v.update_plain(u, tiled, s)) return;
+  if (!plain_off() && !u.gp && v.update_plain(u, tiled, s)) return;
   v.update(u, false, tiled, s);
 }
 
@@ -2089,8 +2089,9 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
     u.gnew = g[cur ^ 1];
     u.gp = (want_gp && i == n - 1) ? gp : nullptr;
     u.beta = betas ? betas[i] : 0.f;
-    // the specialised pass: one algorithm and beta for all, no gp, no idle slots
-    const bool plain = !dm && !u.gp && !u.alg_rec && !u.beta_rec && !u.rec_active && !plain_off();
+    // the specialised pass: ER / HIO / RAAR without gp (per-reconstruction
+    // tables and idle slots in its TAB form)
+    const bool plain = !dm && !u.gp && !plain_off();
     if (!(plain && sh.ex->update_plain(u, sh.t4, s))) sh.ex->update(u, dm, sh.t4, s);
     cur ^= 1;
   }

@@ -602,13 +602,15 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
   }
 }
 
-// The real-space pass for the common case, specialised: ER / HIO / RAAR with
-// one algorithm and beta for every line, no model-before-projection output,
-// no idle reconstructions.  W's cache hint is a template argument (NTW 0 or
-// 3: loads and stores alike); the iterates' hints (nt bits 2, 3) stay
-// arguments.  The same arithmetic as k_row_update, without its optional
-// paths: kept apart, those cost this pass about 0.4 us at 1024^2.
-template <int L, int E, bool T4, int NTW>
+// The real-space pass for the common case, specialised: ER / HIO / RAAR
+// without the model-before-projection output.  W's cache hint is a template
+// argument (NTW 0 or 3: loads and stores alike); the iterates' hints (nt
+// bits 2, 3) stay arguments.  TAB adds what batches of mixed requests need:
+// per-reconstruction algorithm and beta tables and idle reconstructions,
+// handled as k_row_update handles them.  The same arithmetic as
+// k_row_update, without its other optional paths: kept apart, those cost
+// this pass about 0.4 us at 1024^2.
+template <int L, int E, bool T4, int NTW, bool TAB>
 __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs a) {
   using PL = PlanOf<L, E>;
   using RG = RowGeo<L, E, T4>;
@@ -621,9 +623,30 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs
   const int g = RG::line_of(tid), t = RG::pos_of(tid);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int line = blk * M + g;
-  const bool act = line < a.nlines;
+  bool act = line < a.nlines;
+  bool idle_line = false;
+  if constexpr (TAB) {
+    if (a.rec_active) {
+      // idle reconstructions keep their state: W stays in place and the
+      // iterate is carried over to the other ping-pong buffer
+      bool any = false;
+      for (int k = 0; k < M; ++k) {
+        const int l = blk * M + k;
+        if (l < a.nlines && a.rec_active[l / a.lines_per_rec]) any = true;
+      }
+      if (!any) {
+        for (int i = tid; i < M * L; i += RG::NT) {
+          const int l = blk * M + i / L;
+          if (l < a.nlines) a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
+        }
+        return;
+      }
+      idle_line = act && !a.rec_active[line / a.lines_per_rec];
+      act = act && !idle_line;
+    }
+  }
   const View va{shA, g}, vb{shB, g};
-  const int lc = act ? line : a.nlines - 1;
+  const int lc = line < a.nlines ? line : a.nlines - 1;
   const long base = (long)lc * L;
   long wbase = base;
   if constexpr (T4) {
@@ -654,8 +677,13 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs
   load_tw<L, E>(a.tw, t, twr);
   fft_line<L, E, +1>(v, va, vb, t, twr);
   // src/phasing.c:1128-1136 (ER), 1160-1168 (HIO), 1194-1215 (RAAR)
-  const int alg = a.alg;
-  const float beta = a.beta;
+  int alg = a.alg;
+  float beta = a.beta;
+  if constexpr (TAB) {
+    const int rec = act ? line / a.lines_per_rec : 0;
+    if (a.beta_rec) beta = a.beta_rec[rec];
+    if (a.alg_rec) alg = a.alg_rec[rec];
+  }
   const bool er = !(alg == kHIO || alg == kRAAR);
   const float cp = alg == kHIO ? -beta : 1.f - 2.f * beta;
   const float cg = alg == kHIO ? 1.f : beta;
@@ -678,6 +706,9 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs
 #pragma unroll
       for (int q = 0; q < NR; ++q) a.gnew[base + out_pos<L, E>(t, q)] = v[q];
     }
+  } else if (TAB && idle_line) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) a.gnew[base + out_pos<L, E>(t, q)] = g0v[q];
   }
   out_to_in<L, E>(v, va, t);
   __syncthreads();
@@ -994,23 +1025,29 @@ struct LengthOps {
   }
   // the specialised passes; false when the arguments need the general ones
   // (a W hint other than none or loads and stores both)
+  template <bool T4K, bool TAB>
+  static void row_plain_launch(const UpdateArgs& a, int ntw, dim3 grid, dim3 blk, hipStream_t s) {
+    if (ntw) hipLaunchKernelGGL((k_row_plain<L, E, T4K, 3, TAB>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_row_plain<L, E, T4K, 0, TAB>), grid, blk, 0, s, a);
+  }
   static bool update_plain(const UpdateArgs& a0, bool tiled, hipStream_t s) {
     UpdateArgs a = a0;
     a.tw = tw();
     const int ntw = a.nt & 3;
     if (ntw != 0 && ntw != 3) return false;
+    const bool tab = a.alg_rec || a.beta_rec || a.rec_active;
     if (tiled) {
       if constexpr (t4) {
         const dim3 grid((a.nlines + 3) / 4), blk(RowGeo<L, E, true>::NT);
-        if (ntw) hipLaunchKernelGGL((k_row_plain<L, E, true, 3>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_row_plain<L, E, true, 0>), grid, blk, 0, s, a);
+        if (tab) row_plain_launch<true, true>(a, ntw, grid, blk, s);
+        else row_plain_launch<true, false>(a, ntw, grid, blk, s);
         return true;
       }
       return false;
     }
     const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
-    if (ntw) hipLaunchKernelGGL((k_row_plain<L, E, false, 3>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_row_plain<L, E, false, 0>), grid, blk, 0, s, a);
+    if (tab) row_plain_launch<false, true>(a, ntw, grid, blk, s);
+    else row_plain_launch<false, false>(a, ntw, grid, blk, s);
     return true;
   }
   template <bool WIDE>

@@ -343,9 +343,24 @@ class Reconstructor:
                 "fourier_error": ferr, "support_size": ssize}
 
     # -- many reconstructions at once (reference src/_spimage_reconstructor.py:677-740)
+    def _loop_of_singles(self, n, seed, full_output):
+        """reconstruct_loop's output from n single reconstructions, seeds
+        seed .. seed + n - 1."""
+        singles = [self.reconstruct(seed=seed + r) for r in range(n)]
+        out = {"real_space_final": np.stack([o["real_space"][-1] for o in singles]),
+               "fourier_space_final": np.stack([o["fourier_space"][-1] for o in singles]),
+               "support_final": np.stack([o["support"][-1] for o in singles]),
+               "scores_final": {k: np.array([o[k][-1] for o in singles])
+                                for k in ("real_error", "fourier_error", "support_size")}}
+        if full_output:
+            out["single_outputs"] = singles
+        return out
+
     def reconstruct_loop(self, Nrepeats, full_output=False, seed=None):
         """Nrepeats reconstructions from independent random starts, advanced
-        together on the GPU; reconstruction r equals reconstruct(seed=seed + r)."""
+        together on the GPU in one batch (one after another when an axis
+        length has no 2-3-5 plan and runs through Bluestein transforms);
+        reconstruction r equals reconstruct(seed=seed + r)."""
         if not self._ready():
             return None
         A, M, S, supports, phasings = self._prepare()
@@ -353,6 +368,10 @@ class Reconstructor:
         n = int(Nrepeats)
         if seed is None:
             seed = int(np.random.randint(0, 2**31 - 1))
+        if not all(L == 1 or sp.spx_supported_length(int(L)) for L in shape):
+            # an axis without a 2-3-5 plan runs through Bluestein transforms,
+            # which batches do not take: the starts run one after another
+            return self._loop_of_singles(n, int(seed), full_output)
         b = sp.Batch(shape, n)
         try:
             for r in range(n):

@@ -57,6 +57,22 @@ def test_loop_equals_single_reconstructions(sp, shape, kind):
     assert not np.array_equal(loop["real_space_final"][0], loop["real_space_final"][1])
 
 
+def test_loop_on_bluestein_shape(sp):
+    """An axis without a 2-3-5 plan (49 = 7^2): the front-end runs through
+    the Bluestein path and reconstruct_loop falls back to single
+    reconstructions (batches do not take such shapes); the schedule test
+    below checks that path against the oracle."""
+    R = _configure((49, 64), "area")
+    loop = R.reconstruct_loop(2, full_output=True, seed=900)
+    assert loop["real_space_final"].shape == (2, 49, 64)
+    for r in range(2):
+        one = R.reconstruct(seed=900 + r)
+        multi = loop["single_outputs"][r]
+        for key in ("real_space", "support", "fourier_space", "mask"):
+            assert np.array_equal(one[key], multi[key]), (r, key)
+    assert not np.array_equal(loop["real_space_final"][0], loop["real_space_final"][1])
+
+
 def test_reconstruction_converges(sp):
     """A phasing run through the front-end concentrates the model in the
     support: the real-space error falls (at iteration 0 the random-phase
@@ -98,19 +114,19 @@ def _oracle_reconstruct(o, A, M, S, start, total, phasings, supports, out_img, o
     return imgs, sups, np.array(scores)
 
 
-@pytest.mark.parametrize("center", [False, True])
-def test_reconstruct_schedule_matches_oracle(sp, o32, center):
+@pytest.mark.parametrize("shape,center", [((64, 64), False), ((64, 64), True), ((49, 64), False)])
+def test_reconstruct_schedule_matches_oracle(sp, o32, shape, center):
     """reconstruct() against the oracle driven by the reference's loop, from
     the same start model: ER 20, RAAR (beta 0.9 -> 0.8) 10, ER 10; area
     shrinkwrap every 7 iterations (blur 3 -> 1 px, area 0.15 -> 0.10, with
     the centre-image update when `center`) for 28 iterations, then static.
     Algorithm and support switches, Bezier schedules and output points must
     line up: scores within 1e-3 relative, supports equal, images within
-    relative L2 1e-3 at every output point."""
+    relative L2 1e-3 at every output point.  49 x 64 runs the Bluestein
+    path (unfused iterations, 7^2 rows)."""
     from libspimage_amd import workloads
     from libspimage_amd.reconstructor import Reconstructor
     from oracle.oracle import SUP_AREA, SUP_CENTRE, SUP_STATIC
-    shape = (64, 64)
     p = workloads.make_pattern(shape, n_gauss=6, radius=10, photons=1e7, beamstop=2, seed=78)
     R = Reconstructor()
     R.set_intensities(np.fft.fftshift(p.amplitudes.astype(np.float64) ** 2))
