@@ -243,8 +243,9 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // variant's cost depends on where (which XCD's cache) the previous pass left
 // the data.  Both layouts of the Fourier-space arrays are timed (row-major,
 // and 4-column tiled when nx % 4 == 0 and the row variant has a tiled
-// kernel).  The variants (*vx, *vy) are chosen at the shape's first use and
-// serve both layouts; the layout is chosen per shape and per class of batch
+// kernel).  The variants (*vx, *vy) are chosen at the shape's first use,
+// for one reconstruction and for batches together (choose_pair_variants),
+// and serve both layouts; the layout is chosen per shape and per class of batch
 // size (1, 2-3, 4-7, 8 or more reconstructions: in a batch whole
 // reconstructions stay on one XCD, which favours the tiled layout, and more
 // of them fill the chip with wider work-groups).  Layouts and work-group
@@ -259,10 +260,10 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // iteration), batches lose (1024^2 x 16: 212 against 201 us, 750^2 x 16:
 // 209 against 167 us).  Streaming the iterates helps both, by different
 // masks (1024^2: 18.7 us with g_{k+1} stores only; 1024^2 x 16: 184 against
-// 201 us with both iterate streams).  The layout and variants are timed
-// with a default mask per batch class, then the chosen configuration with
-// every W setting (none or both) and iterate setting (none, either, both);
-// SPX_NT=<mask> forces it.  Caller holds tune_mu.
+// 201 us with both iterate streams).  The layouts are timed with the
+// shape's variants and a default mask per batch class, then the chosen
+// layout with every W setting (none or both) and iterate setting (none,
+// either, both); SPX_NT=<mask> forces it.  Caller holds tune_mu.
 struct PairChoice {
   bool t4, wide;
   int nt;
@@ -347,13 +348,146 @@ static void record_shape(int nx, int ny, int nz, const ShapeVariants& sv) {
 
 static int batch_class(int nrec) { return nrec <= 1 ? 0 : nrec < 4 ? 1 : nrec < 8 ? 2 : 3; }
 
+// Tuner scratch for nr reconstructions of a 2-D shape, filled by
+// k_tune_fill, on a stream of its own.
+struct PairScratch {
+  hipStream_t s = nullptr;
+  float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+  float* amp = nullptr;
+  uint8_t* fl = nullptr;
+  bool ok = false;
+  explicit PairScratch(long n) {
+    ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+         hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
+         hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
+         hipMalloc(&fl, (size_t)n) == hipSuccess;
+    if (ok) {
+      hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
+      ok = hipGetLastError() == hipSuccess;
+    }
+  }
+  ~PairScratch() {
+    if (s) (void)hipStreamSynchronize(s);
+    void* bufs[] = {w, g0, g1, amp, fl};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+// Milliseconds per iteration of one configuration of a 2-D shape's passes
+// on nr reconstructions: the column pass then the row pass, in sequence,
+// the iterates ping-ponging as in a run.
+static float time_pair(const PairScratch& b, const Variant& vr, const Variant& vc, int nx, int ny, int nr, bool t4,
+                       bool wd, int ntv) {
+  LineGeom g{nx, nr, (long)nx * ny, t4 ? 4L : (long)nx};
+  if (t4) {
+    g.tshift = 2;
+    g.tstride = 4L * ny;
+  }
+  ColArgs ca{};
+  ca.in = b.w;
+  ca.W = b.w;
+  ca.geo = g;
+  ca.ampx = b.amp;
+  ca.nt = ntv;
+  UpdateArgs u{};
+  u.W = b.w;
+  u.flags = b.fl;
+  u.nlines = nr * ny;
+  u.lines_per_rec = ny;
+  u.alg = kHIO;
+  u.beta = 0.9f;
+  u.scale = 1.f;
+  u.nsize = 1.f;
+  u.nt = ntv;
+  float2 *ga = b.g0, *gb = b.g1;
+  hipStream_t s = b.s;
+  const float ms = time_launches(
+      s,
+      [&] {
+        u.gcur = ga;
+        u.gnew = gb;
+        launch_projection(vc, ca, wd, s);
+        launch_update(vr, u, t4, s);
+        std::swap(ga, gb);
+      },
+      20, 30);
+  if (getenv("SPX_TUNE_LOG"))
+    fprintf(stderr, "tune %d x %d (x%d) %s, hints %d: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
+            t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv, vr.E, vc.E, ms * 1000.f);
+  return ms;
+}
+
+// reconstructions a batch class is timed with: the class of 8 or more with
+// at least 16, the sizes it mostly serves; the scratch is capped at 16 Mi
+// elements (whole reconstructions)
+static int timed_recs(int nrec, long N) {
+  int nr = batch_class(nrec) == 3 && nrec < 16 ? 16 : nrec;
+  if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
+  return nr < 1 ? 1 : nr;
+}
+
+// The variants of a 2-D shape serve every later use of the shape, alone
+// and in batches of any size (so a reconstruction computes the same bits
+// either way), so they are chosen for both: each pair (row E, column E) is
+// timed on one reconstruction and on a batch of 16, in each layout with the
+// class's default hints, and the pair with the least cost wins, cost = the
+// slowdown against the best pair on one reconstruction + half that in the
+// batch.  (900^2: E 8 / 8 is best alone at 23.6 us and E 16 / 16 3.6 %
+// slower, but a batch of 16 takes 222 us with 8 / 8 and 163 us with 16 / 16.)
+static void choose_pair_variants(const LengthEntry* ex, const LengthEntry* ey, int ny, bool t4_shape, int* rc,
+                                 int* cc) {
+  const int nx = ex->L, nvx = ex->nvar, nvy = ey->nvar;
+  const long N = (long)nx * ny;
+  const int nr16 = timed_recs(16, N);
+  const int scales[2] = {1, nr16};
+  const int nscale = nr16 > 1 ? 2 : 1;
+  std::vector<float> t((size_t)2 * nvx * nvy, 1e30f);
+  PairScratch b((long)nr16 * N);
+  if (!b.ok) return;
+  for (int sc = 0; sc < nscale; ++sc) {
+    const int nt0 = sc == 0 ? 3 | 8 : 4 | 8;
+    for (int lay = 0; lay < 2; ++lay) {
+      if (lay == 1 && !t4_shape) continue;
+      for (int r = 0; r < nvx; ++r) {
+        if (lay == 1 && !ex->var[r].t4) continue;
+        for (int c = 0; c < nvy; ++c) {
+          const float ms = time_pair(b, ex->var[r], ey->var[c], nx, ny, scales[sc], lay == 1, false, nt0);
+          float& slot = t[((size_t)sc * nvx + r) * nvy + c];
+          slot = std::min(slot, ms);
+        }
+      }
+    }
+  }
+  float best[2] = {1e30f, 1e30f};
+  for (int sc = 0; sc < nscale; ++sc)
+    for (int i = 0; i < nvx * nvy; ++i) best[sc] = std::min(best[sc], t[(size_t)sc * nvx * nvy + i]);
+  float cost_best = 1e30f;
+  for (int r = 0; r < nvx; ++r)
+    for (int c = 0; c < nvy; ++c) {
+      float cost = t[(size_t)r * nvy + c] / best[0] - 1.f;
+      if (nscale == 2) cost += 0.5f * (t[((size_t)nvx + r) * nvy + c] / best[1] - 1.f);
+      if (cost < cost_best) {
+        cost_best = cost;
+        *rc = r;
+        *cc = c;
+      }
+    }
+  if (getenv("SPX_TUNE_LOG"))
+    fprintf(stderr, "tune %d x %d variants (alone and x%d) chose row E=%d col E=%d\n", nx, ny, nr16, ex->var[*rc].E,
+            ey->var[*cc].E);
+}
+
 static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
                       bool* wide, int* nt) {
   const auto dims = std::make_tuple(ex->L, ny, 1);
   auto known = shape_variants.find(dims);
-  const bool need_row = known == shape_variants.end(), need_col = need_row;
+  const bool need_row = known == shape_variants.end();
   int row_choice = need_row ? default_choice(ex, false) : known->second.v[0];
-  int col_choice = need_col ? default_choice(ey, false, true) : known->second.v[1];
+  int col_choice = need_row ? default_choice(ey, false, true) : known->second.v[1];
+  if (need_row && !tuning_off() && (ex->nvar > 1 || ey->nvar > 1))
+    choose_pair_variants(ex, ey, ny, t4_shape, &row_choice, &col_choice);
   // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows it)
   const char* not4 = getenv("SPX_NO_T4");
   const char* yes4 = getenv("SPX_T4");
@@ -381,119 +515,43 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   const auto key = std::make_tuple(ex->L, ny, batch_class(nrec));
   auto it = tune_layout.find(key);
   if (it != tune_layout.end()) return finish(it->second);
-  // the hints the layouts and variants are timed with
+  // the hints the layouts are timed with
   const int nt0 = batch_class(nrec) == 0 ? 3 | 8 : 4 | 8;
   if (tuning_off()) {
     tune_layout[key] = PairChoice{false, false, nt0};
     return finish(tune_layout[key]);
   }
   const int nx = ex->L;
-  const long N = (long)nx * ny;
-  // the class of 8 or more reconstructions is timed with at least 16, the
-  // sizes it mostly serves; the scratch is capped at 16 Mi elements (whole
-  // reconstructions)
-  int nr = batch_class(nrec) == 3 && nrec < 16 ? 16 : nrec;
-  if ((long)nr * N > (1L << 24)) nr = (int)((1L << 24) / N);
-  if (nr < 1) nr = 1;
-  const long n = (long)nr * N;
-  hipStream_t s = nullptr;
-  float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
-  float* amp = nullptr;
-  uint8_t* fl = nullptr;
-  bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
-            hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
-            hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
-            hipMalloc(&fl, (size_t)n) == hipSuccess;
-  if (ok) {
-    hipLaunchKernelGGL(k_tune_fill, dim3(1024), dim3(256), 0, s, w, g0, g1, amp, fl, n);
-    ok = hipGetLastError() == hipSuccess;
-  }
+  const int nr = timed_recs(nrec, (long)nx * ny);
+  PairScratch b((long)nr * nx * ny);
+  const Variant &vr = ex->var[row_choice], &vc = ey->var[col_choice];
   float best = 1e30f;
-  int br = row_choice, bc = col_choice, blay = 0, bnt = nt0;
+  int blay = 0, bnt = nt0;
   bool bwide = false;
-  // lay 0: row-major, 1: tiled, 2: row-major with the wide projection pass
-  // (only for the best row-major variants, once lay 0 and 1 are timed),
-  // 3..10: the best so far with the eight hint masks other than nt0
+  // with the shape's variants: lay 0 row-major, 1 tiled, 2 row-major with
+  // the wide projection pass, then 3..10 the best of those with the eight
+  // hint masks other than nt0
   static const int kMasks[8] = {0, 3, 4, 7, 8, 11, 12, 15};
-  int r2 = -1, c2 = -1;
-  float best_rm = 1e30f;
-  for (int lay = 0; ok && lay < 11; ++lay) {
+  for (int lay = 0; b.ok && lay < 11; ++lay) {
     const bool flip = lay >= 3;
     const bool t4 = flip ? blay == 1 : lay == 1, wd = flip ? bwide : lay == 2;
     const int ntv = flip ? kMasks[lay - 3] : nt0;
     if (flip && ntv == nt0) continue;
-    if (lay == 1 && !t4_shape) continue;
-    if (lay == 2 && (r2 < 0 || !ey->var[c2].wide)) continue;
-    LineGeom g{nx, nr, N, t4 ? 4L : (long)nx};
-    if (t4) {
-      g.tshift = 2;
-      g.tstride = 4L * ny;
-    }
-    for (int r = 0; r < ex->nvar; ++r) {
-      if ((!need_row && r != row_choice) || (t4 && !ex->var[r].t4) || (lay == 2 && r != r2) || (flip && r != br))
-        continue;
-      for (int c = 0; c < ey->nvar; ++c) {
-        if ((!need_col && c != col_choice) || (lay == 2 && c != c2) || (flip && c != bc)) continue;
-        ColArgs ca{};
-        ca.in = w;
-        ca.W = w;
-        ca.geo = g;
-        ca.ampx = amp;
-        ca.nt = ntv;
-        UpdateArgs u{};
-        u.W = w;
-        u.flags = fl;
-        u.nlines = nr * ny;
-        u.lines_per_rec = ny;
-        u.alg = kHIO;
-        u.beta = 0.9f;
-        u.scale = 1.f;
-        u.nsize = 1.f;
-        u.nt = ntv;
-        const Variant &vr = ex->var[r], &vc = ey->var[c];
-        // steady state, iterates ping-ponging as in a run
-        float2 *ga = g0, *gb = g1;
-        const float ms = time_launches(
-            s,
-            [&] {
-              u.gcur = ga;
-              u.gnew = gb;
-              launch_projection(vc, ca, wd, s);
-              launch_update(vr, u, t4, s);
-              std::swap(ga, gb);
-            },
-            20, 30);
-        if (getenv("SPX_TUNE_LOG"))
-          fprintf(stderr, "tune %d x %d (x%d) %s, hints %d: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
-                  t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv, vr.E, vc.E, ms * 1000.f);
-        if (lay == 0 && ms < best_rm) {
-          best_rm = ms;
-          r2 = r;
-          c2 = c;
-        }
-        if (ms < best) {
-          best = ms;
-          br = r;
-          bc = c;
-          blay = t4 ? 1 : 0;
-          bwide = wd;
-          bnt = ntv;
-        }
-      }
+    if (lay == 1 && !(t4_shape && vr.t4)) continue;
+    if (lay == 2 && !vc.wide) continue;
+    const float ms = time_pair(b, vr, vc, nx, ny, nr, t4, wd, ntv);
+    if (ms < best) {
+      best = ms;
+      blay = t4 ? 1 : 0;
+      bwide = wd;
+      bnt = ntv;
     }
   }
-  row_choice = br;
-  col_choice = bc;
-  (void)hipStreamSynchronize(s);
-  void* bufs[] = {w, g0, g1, amp, fl};
-  for (void* p : bufs)
-    if (p) (void)hipFree(p);
-  if (s) (void)hipStreamDestroy(s);
   tune_layout[key] = PairChoice{blay == 1, bwide, bnt};
   record_tune_line("pair %d %d %d %d %d %d\n", nx, ny, batch_class(nrec), blay == 1 ? 1 : 0, bwide ? 1 : 0, bnt);
   if (getenv("SPX_TUNE_LOG"))
     fprintf(stderr, "tune %d x %d (x%d) chose %s%s, hints %d: row E=%d col E=%d\n", nx, ny, nr,
-            blay == 1 ? "tiled" : "row-major", bwide ? " wide" : "", bnt, ex->var[br].E, ey->var[bc].E);
+            blay == 1 ? "tiled" : "row-major", bwide ? " wide" : "", bnt, vr.E, vc.E);
   return finish(tune_layout[key]);
 }
 
