@@ -112,6 +112,14 @@ int support_update(SpxBatch* b) {
 extern "C" {
 
 SpxBatch* spx_batch_create(int nx, int ny, int nz, int nrec) {
+  // batches run the fused passes only: an axis without a Stockham plan
+  // (Bluestein transforms) is refused for any number of reconstructions
+  const int dims[3] = {nx, ny, nz};
+  for (int d : dims)
+    if (d > 1 && !spx::find_length(d)) {
+      fail(-11, "batches need axis lengths that are products of 2, 3 and 5 up to 4096");
+      return nullptr;
+    }
   spx::Engine* e = spx::Engine::create(nx, ny, nz, nrec);
   if (!e) return nullptr;
   SpxBatch* b = new SpxBatch();

@@ -92,5 +92,7 @@ def test_random_phase_start_any_length(sp):
 
 
 def test_batches_refuse_any_length(sp):
-    assert not bool(sp.lib.spx_batch_create(49, 64, 1, 2))
-    assert "batches need" in sp.last_error()
+    # also with one reconstruction: the unfused path takes no per-slot tables
+    for nrec in (1, 2):
+        assert not bool(sp.lib.spx_batch_create(49, 64, 1, nrec))
+        assert "batches need" in sp.last_error()
