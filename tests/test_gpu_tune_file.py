@@ -29,11 +29,12 @@ print(json.dumps(out))
 """
 
 
-def _child(path):
+def _child(path, **extra):
     env = dict(os.environ, SPX_TUNE_FILE=str(path))
     for k in ("SPX_FFT_E", "SPX_NO_TUNE", "SPX_T4", "SPX_NO_T4", "SPX_COL_WIDE", "SPX_NT"):
         env.pop(k, None)
-    print(f"child process, SPX_TUNE_FILE={path}", flush=True)
+    env.update(extra)
+    print(f"child process, SPX_TUNE_FILE={path} {extra}", flush=True)
     r = subprocess.run([sys.executable, "-c", CHILD, str(ROOT)], env=env, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -68,3 +69,11 @@ def test_choices_are_recorded_and_replayed(tmp_path):
     again = _child(f)
     assert again == first
     assert f.read_text().splitlines() == lines
+
+
+def test_forced_row_and_column_elements(tmp_path):
+    # SPX_FFT_E=<row>,<column> forces the variants (no tuning, nothing recorded)
+    f = tmp_path / "tune.txt"
+    out = _child(f, SPX_FFT_E="16,4", SPX_NT="3")
+    assert out["variant"] == [16, 4] and out["nt"] == 3
+    assert not f.exists() or "shape 256 256 1" not in f.read_text()

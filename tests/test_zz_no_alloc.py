@@ -62,3 +62,10 @@ def test_fixed_array_transform_rejects_too_large(sp):
     a = np.zeros(4, np.complex64)
     assert sp.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 2048, 1025, 1, -1) == -12
     assert "fixed device array" in sp.last_error()
+
+
+def test_fixed_array_transform_rejects_bluestein_lengths(sp):
+    # the chirp tables of lengths without a 2-3-5 plan would be allocated
+    a = np.zeros(49 * 64, np.complex64)
+    assert sp.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 64, 49, 1, -1) == -11
+    assert "not supported" in sp.last_error()
