@@ -225,9 +225,9 @@ int blur_host(const float2* in, float2* out, int nx, int ny, int nz, float sigma
 int fft_device(const float2* in, float2* out, int nx, int ny, int nz, int dir);
 // The one-shot and device transforms accept any axis length up to
 // kBluesteinMaxL besides the Stockham lengths (Bluestein's method over a
-// power-of-two Stockham transform of length <= 4096); the phasing engine
-// needs Stockham lengths.
-constexpr int kBluesteinMaxL = 2048;
+// power-of-two Stockham transform of length <= 8192); single phasers run
+// such shapes with unfused iterations, batches need Stockham lengths.
+constexpr int kBluesteinMaxL = 4096;
 bool fft_length_supported(int L);
 // fft_host staged through a fixed device array instead of an allocation:
 // at most kFixedElems elements (-12 beyond), untuned default plans

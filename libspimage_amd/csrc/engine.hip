@@ -1141,7 +1141,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
     if (dims[a] == 1) continue;
     found[a] = find_length(dims[a]);
     if (!found[a]) {
-      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
+      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 8192)", dims[a]);
       return -11;
     }
     ax[a]->e = found[a];
@@ -1708,7 +1708,7 @@ static int any_path(int nx, int ny, int nz) {
     if (dims[a] > 1 && !find_length(dims[a])) {
       native = false;
       if (!fft_length_supported(dims[a])) {
-        set_error("axis length %d is not supported (products of 2, 3 and 5 up to 4096, any other length up to %d)",
+        set_error("axis length %d is not supported (products of 2, 3 and 5 up to 8192, any other length up to %d)",
                   dims[a], kBluesteinMaxL);
         return -11;
       }
@@ -1739,7 +1739,7 @@ Engine* Engine::create(int nx, int ny, int nz, int nrec) {
   if (path == 1) {
     // an axis without a Stockham plan: one reconstruction, unfused iterations
     if (nrec != 1) {
-      set_error("batches need axis lengths that are products of 2, 3 and 5 up to 4096 (got %d x %d x %d)", nx, ny,
+      set_error("batches need axis lengths that are products of 2, 3 and 5 up to 8192 (got %d x %d x %d)", nx, ny,
                 nz);
       return nullptr;
     }
@@ -2639,8 +2639,11 @@ static int make_shape_fixed(int nx, int ny, int nz, Shape* sh) {
   for (int a = 0; a < 3; ++a) {
     if (dims[a] == 1) continue;
     const LengthEntry* e = find_length(dims[a]);
-    if (!e) {
-      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 4096)", dims[a]);
+    // the fixed array keeps 4096 twiddles per axis: plans of longer lines
+    // need more
+    if (!e || dims[a] > 4096) {
+      set_error("axis length %d is not supported by the fixed-array transform (products of 2, 3 and 5 up to "
+                "4096)", dims[a]);
       return -11;
     }
     ax[a]->e = e;

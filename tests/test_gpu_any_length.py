@@ -37,6 +37,21 @@ def test_one_iteration_any_length(sp, o32, alg, c, shape):
         assert sp.spx_phaser_tiled(gph.ph) == 0
 
 
+# lines beyond 4096: Stockham plans up to 8192 (fused passes), Bluestein up
+# to 4096 (unfused iterations over 8192-point transforms)
+@pytest.mark.parametrize("alg", ["hio", "dm"])
+@pytest.mark.parametrize("shape", [(16, 8192), (6144, 8), (2, 4093)])
+def test_one_iteration_long_lines(sp, o32, alg, shape):
+    obj, amp, meas, sup, g0 = random_case(shape, seed=11)
+    with GpuPhaser(sp, alg, amp, meas, sup, g0, beta=0.9) as gph:
+        assert gph.iterate(1) == 0
+        gam = gph.gammas() if alg == "dm" else (0.0, 0.0)
+        oph = _oracle(o32, alg, amp, meas, sup, g0, beta=0.9, gammas=gam)
+        e = rel(gph.get("model"), oph.g1)
+        print(f"{shape} {alg}: relative L2 {e:.2e} from the FP32 oracle", flush=True)
+        assert e < 1e-5
+
+
 @pytest.mark.parametrize("shape", [(49, 64), (7, 12, 16)])
 def test_er_metrics_any_length(sp, o32, shape):
     obj, amp, meas, sup, g0 = random_case(shape, seed=5)

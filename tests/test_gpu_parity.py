@@ -28,7 +28,9 @@ ONE_ITER_TOL = 1e-5
                                    # 10.9.10, 15.3.15, 9.9.9, 12.5.12, 15.15, 5.15.2.5, 10.10
                                    (1000, 972), (900, 675), (729, 720), (225, 750), (100, 90, 96),
                                    # long lines: two LDS buffers up to 2304, one (in-place stages) from 2400
-                                   (2048, 1536), (4096, 512), (2400, 3072), (2, 4096, 4000)])
+                                   (2048, 1536), (4096, 512), (2400, 3072), (2, 4096, 4000),
+                                   # beyond 4096: 8.8.4.4.8 / 16.8.4.16, 9.9.9.9, 15.5.5.15, 12.9.6.12
+                                   (16, 8192), (8192, 12), (6561, 8), (8, 5625), (3, 7776, 2)])
 def test_fft_matches_oracle(sp, o64, shape):
     rng = np.random.default_rng(sum(shape))
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
@@ -63,14 +65,15 @@ def test_fft_device_entry(sp, o64):
     finally:
         hip.hipFree(d)
     assert rel(out, o64.fft(x, -1)) < 2e-6
-    assert sp.spx_fft_device(None, None, 4099, 4, 1, -1) < 0  # a prime above 2048
+    assert sp.spx_fft_device(None, None, 4099, 4, 1, -1) < 0  # a prime above 4096
     assert "not supported" in sp.last_error()
 
 
 # lengths without a Stockham plan go through Bluestein's method: 7-smooth,
-# primes, mixed with plan lengths, and 2047 (the largest padding, 4096)
+# primes, mixed with plan lengths, 2047 (padding 4096) and 4093, 4095
+# (padding 8192, the largest)
 @pytest.mark.parametrize("shape", [(1, 97), (1, 2047), (49, 64), (448, 30), (13, 11, 17), (7, 1000), (3, 121, 8),
-                                   (30, 7, 2)])
+                                   (30, 7, 2), (1, 4093), (3, 4095), (2053, 4)])
 def test_fft_any_length_matches_oracle(sp, o64, shape):
     rng = np.random.default_rng(sum(shape) + 7)
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
