@@ -176,7 +176,8 @@ def gpu_bench(args, probe_dir=None):
     # over the two passes, is taken off each pass
     gap = max(0.0, (t_f_ev + t_r_ev - t_iter) / 2)
     t_f, t_r = t_f_ev - gap, t_r_ev - gap
-    tiled = sp.spx_phaser_tiled(ph) == 1
+    layout = sp.spx_phaser_tiled(ph)
+    tiled = layout == 1
     nt_hints = sp.spx_phaser_nt_hints(ph)
     before, sup_now = phaser_state(sp, ph)
     save_probe(probe_dir, "hio_1024", "hio", 0.9, p, sup_now, before,
@@ -212,7 +213,8 @@ def gpu_bench(args, probe_dir=None):
         "fft_elements_per_thread": dict(zip(("row", "column"), sp.fft_variant(1024, tiled))),
         "timing": f"median of {SAMPLES} samples of {n_sample} iterations (HIP events on the engine stream)",
         "samples_us_per_iteration": [t * 1e6 for t in samples],
-        "fourier_space_layout": "4-column tiled" if tiled else "row-major",
+        "fourier_space_layout": {1: "4-column tiled", 2: "row-major, transposed Fourier-space pass"}.get(layout,
+                                                                                                   "row-major"),
         "nt_hints": nt_hints,
     }
 

@@ -54,8 +54,10 @@ spimage_EXPORT int spx_fft_variant(int L, int tiled, int *row_elements, int *col
 
 /* 1 when the phaser's engine keeps its Fourier-space work arrays (the
  * row-transformed iterate, amplitudes, margins, phased amplitudes) in the
- * 4-column tiled layout, 0 when row-major, negative without an engine.  The
- * iterates, supports and every array crossing this ABI stay row-major. */
+ * 4-column tiled layout, 0 when row-major, 2 when they stay row-major and
+ * the Fourier-space pass of plain iterations runs on a transposed copy
+ * (large 2-D arrays), negative without an engine.  The iterates, supports
+ * and every array crossing this ABI stay row-major. */
 spimage_EXPORT int spx_phaser_tiled(SpPhaser *ph);
 
 /* Which streams of the phaser's passes carry the non-temporal hint, as the

@@ -1157,7 +1157,7 @@ int spx_phaser_tiled(SpPhaser* ph) {
     spx::set_error("phaser has no HIP engine");
     return -1;
   }
-  return e->tiled() ? 1 : 0;
+  return e->layout();
 }
 
 int spx_phaser_nt_hints(SpPhaser* ph) {

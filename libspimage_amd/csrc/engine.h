@@ -57,6 +57,11 @@ struct Shape {
   int nt = 0;         // non-temporal hints: W loads / stores (bits 0, 1; 2-D only), g_k loads /
                       // g_{k+1} stores (bits 2, 3)
   LineGeom gyt;  // y lines of the tiled layout
+  // 2-D: the Fourier-space pass of plain iterations (the phase objective
+  // without margins, not DM) runs on a transposed copy of W (Wt, columns
+  // contiguous; gytr), made and undone by tiled transposes around it
+  bool tr = false;
+  LineGeom gytr;
 };
 int make_shape(int nx, int ny, int nz, int nrec, Shape* sh);
 // elements per thread (4, 8 or 16; 0 for a trivial axis) of the x, y and z
@@ -91,6 +96,7 @@ class Engine {
   int upload_model(const float2* model, int rec = -1);  // into g1
   int download(const float2* dev, float2* host, int rec = -1);
   bool tiled() const { return sh.t4; }  // Fourier-space arrays in the 4-column tiled layout
+  int layout() const { return sh.tr ? 2 : sh.t4 ? 1 : 0; }  // 0 row-major, 1 tiled, 2 transposed pass
   int nt_hints() const { return sh.nt; }
   float2* g1() { return g[cur]; }
   float2* g0() { return g[cur ^ 1]; }
@@ -186,6 +192,12 @@ class Engine {
   int fft_full(const Shape& shp, const float2* in, float2* out, int dir);
   int iterate_any(const IterParams& p, const float* betas, int n, bool want_gp, hipEvent_t* marks);
   float2* Sany = nullptr;  // Bluestein scratch of the any-length mode
+  // the transposed Fourier-space pass (sh.tr): Wt and the amplitudes in
+  // its layout, refreshed when the amplitudes change
+  int ensure_transposed();
+  float2* Wt = nullptr;
+  float* ampt = nullptr;
+  unsigned amp_version = 0, ampt_version = ~0u;
 
   Shape sh{};
   Shape one{};  // the same shape with one reconstruction (per-slot transforms)

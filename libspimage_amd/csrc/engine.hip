@@ -174,6 +174,59 @@ static void launch_update(const Variant& v, const UpdateArgs& u, bool tiled, hip
   v.update(u, false, tiled, s);
 }
 
+// out = the transpose of each of nrec row-major (rows x cols) arrays,
+// itself row-major (cols x rows): out[c * rows + r] = in[r * cols + c].
+// 32 x 32 tiles through LDS (one padding column against bank conflicts), so
+// both sides move 256-byte row segments of float2.
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ in, T* __restrict__ out, int rows, int cols,
+                                                  long per_rec) {
+  __shared__ T tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const T* src = in + (long)blockIdx.z * per_rec;
+  T* dst = out + (long)blockIdx.z * per_rec;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int r = r0 + ty + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = src[(long)r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int c = c0 + ty + k, r = r0 + tx;
+    if (c < cols && r < rows) dst[(long)c * rows + r] = tile[tx][ty + k];
+  }
+}
+
+template <class T>
+static void enqueue_transpose(const T* in, T* out, int rows, int cols, int nrec, hipStream_t s) {
+  const long per = (long)rows * cols;
+  for (int r0 = 0; r0 < nrec; r0 += 65535) {  // grid z holds at most 65535 reconstructions
+    const dim3 grid((cols + 31) / 32, (rows + 31) / 32, std::min(65535, nrec - r0));
+    hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, s, in + r0 * per, out + r0 * per, rows, cols, per);
+  }
+}
+
+// The Fourier-space pass of a 2-D shape on its transposed copy: W (ny rows
+// of nx) -> Wt (nx columns of ny, each contiguous), the column pass on Wt
+// with the amplitudes in the same layout, and back.  Each step streams
+// whole rows or columns, where the column pass on W reads 32-byte pieces a
+// row apart (one DRAM page each once the arrays outgrow the Infinity Cache).
+static void enqueue_fourier_transposed(const Shape& sh, float2* W, float2* Wt, const float* ampt, ColArgs a,
+                                       hipStream_t s) {
+  enqueue_transpose(W, Wt, sh.ny, sh.nx, sh.nrec, s);
+  a.in = Wt;
+  a.W = Wt;
+  a.ampx = ampt;
+  a.geo = sh.gytr;
+  a.outer_per_rec = 1;
+  // contiguous lines: the row passes' work-groups (several per CU); the
+  // column kernel on the same geometry as the fallback for other hints
+  if (!sh.ey->line_project(a, s)) sh.ey->col(kColProject, a, s);
+  enqueue_transpose(Wt, W, sh.nx, sh.ny, sh.nrec, s);
+}
+
 // Single-pass tuning of one axis outside a 2-D pair (1-D and 3-D shapes):
 // times the pass's hot kernel on zero-filled scratch of the real geometry
 // and returns the variant index.  Caller holds tune_mu.
@@ -261,13 +314,18 @@ static int tune_length(const LengthEntry* e, int lines, const LineGeom* col_geo)
 // 209 against 167 us).  Streaming the iterates helps both, by different
 // masks (1024^2: 18.7 us with g_{k+1} stores only; 1024^2 x 16: 184 against
 // 201 us with both iterate streams).  The layouts are timed with the
-// shape's variants and a default mask per batch class, then the chosen
-// layout with every W setting (none or both) and iterate setting (none,
-// either, both); SPX_NT=<mask> forces it.  Caller holds tune_mu.
+// shape's variants, a default mask per batch class and that mask with W's
+// setting flipped, then the best with every W setting (none or both) and
+// iterate setting (none, either, both); SPX_NT=<mask> forces it.  Large
+// arrays may also run the Fourier-space pass on a transposed copy of W
+// (kLayTransposed, enqueue_fourier_transposed).  Caller holds tune_mu.
 struct PairChoice {
   bool t4, wide;
   int nt;
+  bool tr = false;  // the transposed Fourier-space pass
 };
+// layouts of a 2-D shape's Fourier-space pass
+enum PairLayout { kLayRowMajor = 0, kLayTiled = 1, kLayTransposed = 2 };
 static std::map<std::tuple<int, int, int>, PairChoice> tune_layout;
 // iterate hints chosen per 3-D shape and batch class (tune_volume_hints)
 static std::map<std::tuple<int, int, int, int>, int> volume_hints;
@@ -277,7 +335,7 @@ static std::map<std::tuple<int, int, int, int>, int> volume_hints;
 // before the process's first choice, so a later process (a restarted server,
 // a profiler run) makes the same choices without timing anything:
 //   shape <nx> <ny> <nz> <Ex> <Ey> <Ez>        elements per thread (0: axis of length 1)
-//   pair <nx> <ny> <class> <tiled> <wide> <hints>
+//   pair <nx> <ny> <class> <layout> <wide> <hints>   (layout 0 row-major, 1 tiled, 2 transposed pass)
 //   volume <nx> <ny> <nz> <class> <hints>
 // (class: batch class 0-3, hints: the non-temporal mask).  Lines that name a
 // variant this build lacks, or that do not parse, are skipped.  Caller holds
@@ -321,7 +379,8 @@ static void load_tune_file() {
       }
       if (ok) shape_variants[std::make_tuple(a[0], a[1], a[2])] = sv;
     } else if (sscanf(line, "pair %d %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) == 6) {
-      tune_layout[std::make_tuple(a[0], a[1], a[2])] = PairChoice{a[3] != 0, a[4] != 0, a[5] & 15};
+      tune_layout[std::make_tuple(a[0], a[1], a[2])] =
+          PairChoice{a[3] == kLayTiled, a[4] != 0, a[5] & 15, a[3] == kLayTransposed};
     } else if (sscanf(line, "volume %d %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4]) == 5) {
       volume_hints[std::make_tuple(a[0], a[1], a[2], a[3])] = a[4] & 15;
     }
@@ -353,10 +412,16 @@ static int batch_class(int nrec) { return nrec <= 1 ? 0 : nrec < 4 ? 1 : nrec < 
 struct PairScratch {
   hipStream_t s = nullptr;
   float2 *w = nullptr, *g0 = nullptr, *g1 = nullptr;
+  float2* wt = nullptr;  // the transposed pass's copy, on first use
   float* amp = nullptr;
   uint8_t* fl = nullptr;
+  long n = 0;
   bool ok = false;
-  explicit PairScratch(long n) {
+  bool ensure_wt() {
+    if (!wt && hipMalloc(&wt, sizeof(float2) * n) != hipSuccess) wt = nullptr;
+    return wt != nullptr;
+  }
+  explicit PairScratch(long n_) : n(n_) {
     ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
          hipMalloc(&w, sizeof(float2) * n) == hipSuccess && hipMalloc(&g0, sizeof(float2) * n) == hipSuccess &&
          hipMalloc(&g1, sizeof(float2) * n) == hipSuccess && hipMalloc(&amp, sizeof(float) * n) == hipSuccess &&
@@ -368,7 +433,7 @@ struct PairScratch {
   }
   ~PairScratch() {
     if (s) (void)hipStreamSynchronize(s);
-    void* bufs[] = {w, g0, g1, amp, fl};
+    void* bufs[] = {w, g0, g1, amp, fl, wt};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
@@ -378,13 +443,18 @@ struct PairScratch {
 // Milliseconds per iteration of one configuration of a 2-D shape's passes
 // on nr reconstructions: the column pass then the row pass, in sequence,
 // the iterates ping-ponging as in a run.
-static float time_pair(const PairScratch& b, const Variant& vr, const Variant& vc, int nx, int ny, int nr, bool t4,
-                       bool wd, int ntv) {
+static float time_pair(PairScratch& b, const Variant& vr, const Variant& vc, int nx, int ny, int nr, int lay, bool wd,
+                       int ntv) {
+  const bool t4 = lay == kLayTiled, trp = lay == kLayTransposed;
+  if (trp && !b.ensure_wt()) return 1e30f;
   LineGeom g{nx, nr, (long)nx * ny, t4 ? 4L : (long)nx};
   if (t4) {
     g.tshift = 2;
     g.tstride = 4L * ny;
   }
+  LineGeom gt{nx, nr, (long)nx * ny, 1L};
+  gt.tshift = 0;
+  gt.tstride = ny;
   ColArgs ca{};
   ca.in = b.w;
   ca.W = b.w;
@@ -408,14 +478,24 @@ static float time_pair(const PairScratch& b, const Variant& vr, const Variant& v
       [&] {
         u.gcur = ga;
         u.gnew = gb;
-        launch_projection(vc, ca, wd, s);
+        if (trp) {
+          enqueue_transpose(b.w, b.wt, ny, nx, nr, s);
+          ColArgs ct = ca;
+          ct.in = b.wt;
+          ct.W = b.wt;
+          ct.geo = gt;
+          if (!vc.line_project(ct, s)) vc.col(kColProject, ct, s);
+          enqueue_transpose(b.wt, b.w, nx, ny, nr, s);
+        } else {
+          launch_projection(vc, ca, wd, s);
+        }
         launch_update(vr, u, t4, s);
         std::swap(ga, gb);
       },
       20, 30);
   if (getenv("SPX_TUNE_LOG"))
     fprintf(stderr, "tune %d x %d (x%d) %s, hints %d: row E=%d col E=%d  %.2f us\n", nx, ny, nr,
-            t4 ? "tiled" : (wd ? "row-major wide" : "row-major"), ntv, vr.E, vc.E, ms * 1000.f);
+            t4 ? "tiled" : trp ? "transposed" : (wd ? "row-major wide" : "row-major"), ntv, vr.E, vc.E, ms * 1000.f);
   return ms;
 }
 
@@ -453,7 +533,7 @@ static void choose_pair_variants(const LengthEntry* ex, const LengthEntry* ey, i
       for (int r = 0; r < nvx; ++r) {
         if (lay == 1 && !ex->var[r].t4) continue;
         for (int c = 0; c < nvy; ++c) {
-          const float ms = time_pair(b, ex->var[r], ey->var[c], nx, ny, scales[sc], lay == 1, false, nt0);
+          const float ms = time_pair(b, ex->var[r], ey->var[c], nx, ny, scales[sc], lay, false, nt0);
           float& slot = t[((size_t)sc * nvx + r) * nvy + c];
           slot = std::min(slot, ms);
         }
@@ -479,8 +559,8 @@ static void choose_pair_variants(const LengthEntry* ex, const LengthEntry* ey, i
             ey->var[*cc].E);
 }
 
-static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
-                      bool* wide, int* nt) {
+static int tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int nrec, bool t4_shape, int* vx, int* vy,
+                     bool* wide, int* nt) {
   const auto dims = std::make_tuple(ex->L, ny, 1);
   auto known = shape_variants.find(dims);
   const bool need_row = known == shape_variants.end();
@@ -488,18 +568,14 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   int col_choice = need_row ? default_choice(ey, false, true) : known->second.v[1];
   if (need_row && !tuning_off() && (ex->nvar > 1 || ey->nvar > 1))
     choose_pair_variants(ex, ey, ny, t4_shape, &row_choice, &col_choice);
-  // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows it)
+  // SPX_NO_T4=1 / SPX_T4=1 force the layout (when the row variant allows
+  // it), SPX_TR=1 / 0 the transposed pass
   const char* not4 = getenv("SPX_NO_T4");
   const char* yes4 = getenv("SPX_T4");
-  auto decided = [&](bool t4) {
-    const bool possible = t4_shape && ex->var[row_choice].t4;
-    if (not4 && atoi(not4) == 1) return false;
-    if (yes4 && atoi(yes4) == 1) return possible;
-    return t4 && possible;
-  };
+  const char* trenv = getenv("SPX_TR");
   const char* wenv = getenv("SPX_COL_WIDE");
   const char* ntenv = getenv("SPX_NT");
-  auto finish = [&](PairChoice pc) {
+  auto finish = [&](PairChoice pc) -> int {
     *nt = ntenv ? (atoi(ntenv) & 15) : pc.nt;
     if (need_row) {
       shape_variants[dims] = ShapeVariants{{row_choice, col_choice, 0}};
@@ -507,10 +583,16 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
     }
     *vx = row_choice;
     *vy = col_choice;
-    const bool t4 = decided(pc.t4);
-    bool w = wenv ? atoi(wenv) == 1 : pc.wide;
-    *wide = !t4 && w && ey->var[col_choice].wide;
-    return t4;
+    const bool forced4 = (not4 && atoi(not4) == 1) || (yes4 && atoi(yes4) == 1);
+    const bool tr = trenv ? atoi(trenv) == 1 : (pc.tr && !forced4);
+    const bool possible = t4_shape && ex->var[row_choice].t4;
+    bool t4 = pc.t4 && possible;
+    if (not4 && atoi(not4) == 1) t4 = false;
+    if (yes4 && atoi(yes4) == 1) t4 = possible;
+    t4 = t4 && !tr;
+    const bool w = wenv ? atoi(wenv) == 1 : pc.wide;
+    *wide = !t4 && !tr && w && ey->var[col_choice].wide;
+    return tr ? kLayTransposed : t4 ? kLayTiled : kLayRowMajor;
   };
   const auto key = std::make_tuple(ex->L, ny, batch_class(nrec));
   auto it = tune_layout.find(key);
@@ -526,32 +608,40 @@ static bool tune_pair(const LengthEntry* ex, const LengthEntry* ey, int ny, int 
   PairScratch b((long)nr * nx * ny);
   const Variant &vr = ex->var[row_choice], &vc = ey->var[col_choice];
   float best = 1e30f;
-  int blay = 0, bnt = nt0;
+  int blay = kLayRowMajor, bnt = nt0;
   bool bwide = false;
-  // with the shape's variants: lay 0 row-major, 1 tiled, 2 row-major with
-  // the wide projection pass, then 3..10 the best of those with the eight
-  // hint masks other than nt0
+  // with the shape's variants: 0 row-major, 1 tiled, 2 row-major with the
+  // wide projection pass, 3 the transposed pass (from 4 Mi pixels per
+  // launch, where the arrays leave the Infinity Cache), each with W's hint
+  // as in nt0 and flipped (the hint that helps 1024^2 costs 3072^2 half
+  // again: 159 against 243 us per iteration), then 8..15 the best of those
+  // with the eight hint masks
   static const int kMasks[8] = {0, 3, 4, 7, 8, 11, 12, 15};
-  for (int lay = 0; b.ok && lay < 11; ++lay) {
-    const bool flip = lay >= 3;
-    const bool t4 = flip ? blay == 1 : lay == 1, wd = flip ? bwide : lay == 2;
-    const int ntv = flip ? kMasks[lay - 3] : nt0;
-    if (flip && ntv == nt0) continue;
-    if (lay == 1 && !(t4_shape && vr.t4)) continue;
-    if (lay == 2 && !vc.wide) continue;
-    const float ms = time_pair(b, vr, vc, nx, ny, nr, t4, wd, ntv);
+  const bool try_tr = (long)nx * ny * nr >= (1L << 22) || (trenv && atoi(trenv) == 1);
+  for (int cand = 0; b.ok && cand < 16; ++cand) {
+    const bool flip = cand >= 8;
+    const int c4 = cand & 3;
+    const int lay = flip ? blay : c4 == 1 ? kLayTiled : c4 == 3 ? kLayTransposed : kLayRowMajor;
+    const bool wd = flip ? bwide : c4 == 2;
+    const int ntv = flip ? kMasks[cand - 8] : (cand < 4 ? nt0 : nt0 ^ 3);
+    if (flip && (ntv == nt0 || ntv == (nt0 ^ 3))) continue;
+    if (!flip && c4 == 1 && !(t4_shape && vr.t4)) continue;
+    if (!flip && c4 == 2 && !vc.wide) continue;
+    if (!flip && c4 == 3 && !try_tr) continue;
+    const float ms = time_pair(b, vr, vc, nx, ny, nr, lay, wd, ntv);
     if (ms < best) {
       best = ms;
-      blay = t4 ? 1 : 0;
+      blay = lay;
       bwide = wd;
       bnt = ntv;
     }
   }
-  tune_layout[key] = PairChoice{blay == 1, bwide, bnt};
-  record_tune_line("pair %d %d %d %d %d %d\n", nx, ny, batch_class(nrec), blay == 1 ? 1 : 0, bwide ? 1 : 0, bnt);
+  tune_layout[key] = PairChoice{blay == kLayTiled, bwide, bnt, blay == kLayTransposed};
+  record_tune_line("pair %d %d %d %d %d %d\n", nx, ny, batch_class(nrec), blay, bwide ? 1 : 0, bnt);
   if (getenv("SPX_TUNE_LOG"))
     fprintf(stderr, "tune %d x %d (x%d) chose %s%s, hints %d: row E=%d col E=%d\n", nx, ny, nr,
-            blay == 1 ? "tiled" : "row-major", bwide ? " wide" : "", bnt, vr.E, vc.E);
+            blay == kLayTiled ? "tiled" : blay == kLayTransposed ? "transposed" : "row-major", bwide ? " wide" : "",
+            bnt, vr.E, vc.E);
   return finish(tune_layout[key]);
 }
 
@@ -1155,10 +1245,16 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
   sh->gyt = LineGeom{nx, nrec, (long)nx * ny, 4L};
   sh->gyt.tshift = 2;
   sh->gyt.tstride = 4L * ny;
+  // columns of the transposed copy: column x of reconstruction r at r N + x ny
+  sh->gytr = LineGeom{nx, nrec, (long)nx * ny, 1L};
+  sh->gytr.tshift = 0;
+  sh->gytr.tstride = ny;
   std::lock_guard<std::mutex> lock(tune_mu);
   load_tune_file();
   if (found[0] && found[1] && !found[2]) {
-    sh->t4 = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw, &sh->nt);
+    const int lay = tune_pair(found[0], found[1], ny, nrec, nx % 4 == 0, &sh->ex.v, &sh->ey.v, &sh->colw, &sh->nt);
+    sh->t4 = lay == kLayTiled;
+    sh->tr = lay == kLayTransposed;
     return 0;
   }
   const auto key = std::make_tuple(nx, ny, nz);
@@ -1803,7 +1899,7 @@ Engine::~Engine() {
   if (s) (void)hipStreamSynchronize(s);
   drop_graphs();
   void* ptrs[] = {g[0], g[1], W,        Wf,  gp,   S1,   ampx,    amp_min,  amp_max,   absb,
-                  phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active, Sany};
+                  phased, flags, partials, red, hist, ctl, alg_rec, beta_rec, rec_active, Sany, Wt, ampt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : templates) (void)hipFree(e.second);
@@ -1823,6 +1919,7 @@ int Engine::upload_amplitudes(const float* amp, const uint8_t* fl, int rec) {
   SPX_HIP(hipGetLastError());
   SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
+  ++amp_version;
   return 0;
 }
 
@@ -2007,6 +2104,17 @@ int Engine::finish_model_init(int fl, int rec) {
   return 0;
 }
 
+int Engine::ensure_transposed() {
+  if (!Wt) SPX_HIP(hipMalloc(&Wt, sizeof(float2) * NT));
+  if (!ampt) SPX_HIP(hipMalloc(&ampt, sizeof(float) * NT));
+  if (ampt_version != amp_version) {
+    enqueue_transpose(ampx, ampt, ny, nx, nrec, s);
+    SPX_HIP(hipGetLastError());
+    ampt_version = amp_version;
+  }
+  return 0;
+}
+
 int Engine::ensure_R() {
   if (any_mode) return 0;  // unfused iterations transform the iterate in full
   if (!R_valid) {
@@ -2138,9 +2246,14 @@ int Engine::iterate_impl(const IterParams& p, const float* betas, int n, bool wa
   u.nsize = (float)N;
   u.nt = sh.nt;
   u.rec_active = some_idle ? rec_active : nullptr;
+  // the transposed Fourier-space pass serves the plain projection only
+  const bool trp = sh.tr && !dm && plain_projection(kColProject, ca);
+  if (trp)
+    if (int r = ensure_transposed()) return r;
   for (int i = 0; i < n; ++i) {
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i], s));
-    enqueue_fourier(sh, kColProject, W, W, ca, s, sh.t4);
+    if (trp) enqueue_fourier_transposed(sh, W, Wt, ampt, ca, s);
+    else enqueue_fourier(sh, kColProject, W, W, ca, s, sh.t4);
     if (dm) enqueue_fourier(sh, kColProject, Wf, Wf, caf, s, sh.t4);
     if (marks) SPX_HIP(hipEventRecord(marks[2 * i + 1], s));
     u.gcur = g[cur];

@@ -982,6 +982,59 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col_plain(ColArg
   }
 }
 
+// The Fourier-space pass on contiguous lines (the transposed copy of W of a
+// 2-D shape, whose columns are rows of L = ny elements): k_col_plain's
+// arithmetic with the row passes' geometry, RowShape<L, E>::M lines per
+// work-group, so that several work-groups share a CU and overlap their
+// phases.  Lines l = 0 .. ncontig * nouter - 1 start at l * L; line l
+// belongs to reconstruction l / ncontig; idle reconstructions' lines are
+// transformed but not stored.
+template <int L, int E, int NTW>
+__global__ __launch_bounds__((RowShape<L, E>::NT)) void k_line_project(ColArgs a) {
+  using PL = PlanOf<L, E>;
+  using RG = RowGeo<L, E, false>;
+  constexpr int M = RG::M;
+  constexpr int NR = PL::NREG;
+  using View = LdsView<L, M, false>;
+  __shared__ float2 shA[View::SIZE];
+  __shared__ float2 shB[SingleBuf<L>::value ? 1 : View::SIZE];
+  const int tid = threadIdx.x;
+  const int g = RG::line_of(tid), t = RG::pos_of(tid);
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int nlines = a.geo.ncontig * a.geo.nouter;
+  const int line = blk * M + g;
+  const int lc = line < nlines ? line : nlines - 1;
+  const bool act = line < nlines && !(a.rec_active && !a.rec_active[lc / a.geo.ncontig]);
+  const long base = (long)lc * L;
+  const View va{shA, g}, vb{shB, g};
+  float2 v[NR];
+  float ampv[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const float2* p = &a.in[base + in_pos<L, E>(t, q)];
+    v[q] = NTW ? ld_nt(p) : *p;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) ampv[q] = a.ampx[base + out_pos<L, E>(t, q)];
+  TwRegs<L, E> twr;
+  load_tw<L, E>(a.tw, t, twr);
+  fft_line<L, E, -1>(v, va, vb, t, twr);
+  const bool centro = a.centro != 0;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = project_phase(v[q], ampv[q], centro);
+  out_to_in<L, E>(v, va, t);
+  __syncthreads();
+  fft_line<L, E, +1>(v, va, vb, t, twr);
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      float2* p = &a.W[base + out_pos<L, E>(t, q)];
+      if constexpr (NTW != 0) st_nt(p, v[q]);
+      else *p = v[q];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers and the per-length dispatch table.
 // ---------------------------------------------------------------------------
@@ -1048,6 +1101,19 @@ struct LengthOps {
     const dim3 grid((a.nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
     if (tab) row_plain_launch<false, true>(a, ntw, grid, blk, s);
     else row_plain_launch<false, false>(a, ntw, grid, blk, s);
+    return true;
+  }
+  // the projection pass on contiguous lines (k_line_project); false when
+  // W's hint is neither none nor loads and stores both
+  static bool line_project(const ColArgs& a0, hipStream_t s) {
+    ColArgs a = a0;
+    a.tw = a0.tw ? a0.tw : tw();
+    const int ntw = a.nt & 3;
+    if (ntw != 0 && ntw != 3) return false;
+    const int nlines = a.geo.ncontig * a.geo.nouter;
+    const dim3 grid((nlines + RowShape<L, E>::M - 1) / RowShape<L, E>::M), blk(RowShape<L, E>::NT);
+    if (ntw) hipLaunchKernelGGL((k_line_project<L, E, 3>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_line_project<L, E, 0>), grid, blk, 0, s, a);
     return true;
   }
   template <bool WIDE>
@@ -1120,6 +1186,7 @@ struct Variant {
   void (*col_project_wide)(const ColArgs&, hipStream_t);
   bool (*update_plain)(const UpdateArgs&, bool, hipStream_t);
   bool (*col_project_plain)(const ColArgs&, bool, hipStream_t);
+  bool (*line_project)(const ColArgs&, hipStream_t);
 };
 
 template <int L, int E>
@@ -1134,7 +1201,8 @@ constexpr Variant variant() {
                  LengthOps<L, E>::wide,
                  &LengthOps<L, E>::col_project_wide,
                  &LengthOps<L, E>::update_plain,
-                 &LengthOps<L, E>::col_project_plain};
+                 &LengthOps<L, E>::col_project_plain,
+                 &LengthOps<L, E>::line_project};
 }
 
 // Dispatch table entry for one length: its compiled variants.
@@ -1159,6 +1227,7 @@ struct AxisOps {
   void col(int mode, const ColArgs& a, hipStream_t s) const { var().col(mode, a, s); }
   void col_project_wide(const ColArgs& a, hipStream_t s) const { var().col_project_wide(a, s); }
   bool update_plain(const UpdateArgs& a, bool tiled, hipStream_t s) const { return var().update_plain(a, tiled, s); }
+  bool line_project(const ColArgs& a, hipStream_t s) const { return var().line_project(a, s); }
   bool col_project_plain(const ColArgs& a, bool wide, hipStream_t s) const {
     return var().col_project_plain(a, wide, s);
   }
