@@ -2295,6 +2295,9 @@ int Engine::iterate_graph(const IterParams& p, float beta, int n) {
   if (int r = ensure_R()) return r;
   if (p.alg == kDiffMap)
     if (int r = ensure_Rf(p.gamma1)) return r;
+  // allocations and the amplitudes' transpose stay out of the capture
+  if (sh.tr)
+    if (int r = ensure_transposed()) return r;
   const bool same = gvalid && gkey.alg == p.alg && gkey.constraints == p.constraints &&
                     gkey.objective == p.objective && gkey.gamma1 == p.gamma1 && gkey.gamma2 == p.gamma2 &&
                     gbeta == beta;

@@ -3,8 +3,9 @@ the column pass runs on contiguous columns, and W is transposed back) must
 compute the same bits as the row-major pass.  The layout is fixed per shape
 at its first use, so each side runs in a child process with the layout
 forced (SPX_TR=1, or SPX_TR=0 with SPX_NO_T4=1), on non-square shapes so a
-swapped index shows, for a single phaser with shrinkwrap and for a batch
-with per-reconstruction algorithms and an idle slot."""
+swapped index shows, for a single phaser with shrinkwrap, for a batch with
+per-reconstruction algorithms and an idle slot, and for graph replays of a
+batch against its eager iterations."""
 import os
 import subprocess
 import sys
@@ -51,6 +52,22 @@ b.iterate(6)
 res["model_batch"] = b.model()
 res["ef_batch"] = b.efourier()[0]
 b.close()
+# graph replays (captured after the transposed pass's set-up) against eager
+hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+def graph_batch(graphs):
+    g = sp.Batch(bshape, 3)
+    for r, (_, a, m, s_, _) in enumerate(pats[:3]):
+        g.set_data(r, a, m, s_)
+    g.init_model(sp.SpModelRandomPhases, 9)
+    g.set_algorithm(hio)
+    g.use_graphs(graphs)
+    for chunk in (5, 9, 4):
+        g.iterate(chunk)
+        g.synchronize()
+    m_ = g.model()
+    g.close()
+    return m_
+res["graph_equal"] = np.array(np.array_equal(graph_batch(True), graph_batch(False)))
 np.savez(out, **res)
 """
 
@@ -76,3 +93,4 @@ def test_transposed_pass_equals_row_major(tmp_path):
     assert int(tr["layout_single"]) == 2 and int(rm["layout_single"]) == 0
     for key in ("model_single", "ef_single", "model_batch", "ef_batch"):
         assert np.array_equal(tr[key], rm[key]), key
+    assert bool(tr["graph_equal"]) and bool(rm["graph_equal"])
