@@ -6,10 +6,11 @@
  * host-to-device one-shots sp_image_cuda_fft/ifft (include/spimage/fft.h:129-132,
  * src/fft.cu:7-43).  All of those names are exported here and share one
  * implementation: unnormalised c2c over (z,y,x), forward sign -1.
- * Supported lengths per axis: products of 2, 3 and 5 up to 8192 (Stockham
- * passes; 6250 excepted), and any other length up to 4096 (Bluestein's
- * method over a power-of-two Stockham transform); beyond that the functions
- * print an error and return NULL / leave the output untouched.
+ * Supported lengths per axis: products of 2, 3, 5 and 7 up to 8192 (Stockham
+ * passes; 5250, 6250, 7350, 7560 and 7840 excepted), and any other length up
+ * to 4096 (Bluestein's method over a power-of-two Stockham transform);
+ * beyond that the functions print an error and return NULL / leave the
+ * output untouched.
  */
 #ifndef SPX_SP_FFT_H
 #define SPX_SP_FFT_H 1

@@ -80,8 +80,9 @@ spimage_EXPORT int spx_fft_device(void *d_in, void *d_out, int nx, int ny, int n
  * fixed device array of 2^21 complex values (16 MiB, plus 96 KiB of twiddle
  * tables) that each device holds for the life of the process, so larger
  * arrays are rejected with -12; it uses the default transform plans, not
- * tuned ones, and takes axis lengths that are products of 2, 3 and 5 up to
- * 4096 (-11 otherwise: the Bluestein tables of other lengths are allocated).
+ * tuned ones, and takes axis lengths that are products of 2, 3, 5 and 7 up
+ * to 4096 (-11 otherwise: the Bluestein tables of other lengths are
+ * allocated).
  * It synchronises before returning.  Calls on one device share the fixed
  * array: calls that overlap on one device race. */
 spimage_EXPORT int spx_fft_noalloc(const float *in, float *out, int nx, int ny, int nz, int direction);

@@ -115,7 +115,8 @@ spimage_EXPORT void sp_phaser_set_objective(SpPhaser *ph, SpPhasingObjective obj
  *   -9  when SpEngineCPU is requested (this library has no CPU engine)
  *   -10 when no HIP device is present (the reference would fall back to CPU)
  * and sp_phaser_iterate returns -11 when the HIP engine cannot serve the
- * array shape (a length that is not a product of 2, 3 and 5, or a 1-D array). */
+ * array shape (an axis length above 4096 without a Stockham plan, or a 1-D
+ * array). */
 spimage_EXPORT int sp_phaser_init(SpPhaser *ph, SpPhasingAlgorithm *alg, SpSupportArray *sup_alg, SpPhasingEngine engine);
 spimage_EXPORT int sp_phaser_set_phasing_algorithm(SpPhaser *ph, SpPhasingAlgorithm *alg);
 /* defined but not declared by the reference (src/phasing.c:597-604) */

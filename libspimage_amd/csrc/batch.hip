@@ -117,7 +117,7 @@ SpxBatch* spx_batch_create(int nx, int ny, int nz, int nrec) {
   const int dims[3] = {nx, ny, nz};
   for (int d : dims)
     if (d > 1 && !spx::find_length(d)) {
-      fail(-11, "batches need axis lengths that are products of 2, 3 and 5 up to 8192");
+      fail(-11, "batches need axis lengths that are products of 2, 3, 5 and 7 up to 8192");
       return nullptr;
     }
   spx::Engine* e = spx::Engine::create(nx, ny, nz, nrec);

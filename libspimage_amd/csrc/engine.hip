@@ -1231,7 +1231,7 @@ int make_shape(int nx, int ny, int nz, int nrec, Shape* sh) {
     if (dims[a] == 1) continue;
     found[a] = find_length(dims[a]);
     if (!found[a]) {
-      set_error("axis length %d is not supported (supported: products of 2, 3 and 5 up to 8192)", dims[a]);
+      set_error("axis length %d is not supported (supported: products of 2, 3, 5 and 7 up to 8192)", dims[a]);
       return -11;
     }
     ax[a]->e = found[a];
@@ -1468,7 +1468,7 @@ static int last_axis_tiles_per_rec(const Shape& sh) {
 // Transforms of any axis length up to kBluesteinMaxL (Bluestein's chirp-z
 // method), for the Image-level and device-buffer transforms: FFTW plans any
 // length (reference src/fft.c:79,95,163,185), the Stockham passes only
-// products of 2, 3 and 5.  With w[n] = exp(i pi n^2 / L),
+// products of 2, 3, 5 and 7.  With w[n] = exp(i pi n^2 / L),
 //   X[k] = conj(w[k]) sum_n (x[n] conj(w[n])) w[k - n],
 // a circular convolution of length M >= 2L - 1 (a power of two, so the
 // Stockham passes transform it): zero-padded a = x conj(w), B = FFT_M(b)/M
@@ -1804,7 +1804,7 @@ static int any_path(int nx, int ny, int nz) {
     if (dims[a] > 1 && !find_length(dims[a])) {
       native = false;
       if (!fft_length_supported(dims[a])) {
-        set_error("axis length %d is not supported (products of 2, 3 and 5 up to 8192, any other length up to %d)",
+        set_error("axis length %d is not supported (products of 2, 3, 5 and 7 up to 8192, any other length up to %d)",
                   dims[a], kBluesteinMaxL);
         return -11;
       }
@@ -1835,7 +1835,7 @@ Engine* Engine::create(int nx, int ny, int nz, int nrec) {
   if (path == 1) {
     // an axis without a Stockham plan: one reconstruction, unfused iterations
     if (nrec != 1) {
-      set_error("batches need axis lengths that are products of 2, 3 and 5 up to 8192 (got %d x %d x %d)", nx, ny,
+      set_error("batches need axis lengths that are products of 2, 3, 5 and 7 up to 8192 (got %d x %d x %d)", nx, ny,
                 nz);
       return nullptr;
     }
@@ -2758,7 +2758,7 @@ static int make_shape_fixed(int nx, int ny, int nz, Shape* sh) {
     // the fixed array keeps 4096 twiddles per axis: plans of longer lines
     // need more
     if (!e || dims[a] > 4096) {
-      set_error("axis length %d is not supported by the fixed-array transform (products of 2, 3 and 5 up to "
+      set_error("axis length %d is not supported by the fixed-array transform (products of 2, 3, 5 and 7 up to "
                 "4096)", dims[a]);
       return -11;
     }

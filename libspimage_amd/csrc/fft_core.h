@@ -6,8 +6,8 @@
 // (X_k = sum_n x_n e^{-2 pi i k n / L}), inverse sign +1, row-major with x
 // fastest.
 //
-// A length-L line is split into radix stages (2, 3, 4, 5, 6, 8, 9, 10, 12,
-// 15, 16) chosen at compile time.  T threads cooperate on a line.  Stage 0 reads its inputs from
+// A length-L line is split into radix stages (2, 3, 4, 5, 6, 7, 8, 9, 10,
+// 12, 15, 16) chosen at compile time.  T threads cooperate on a line.  Stage 0 reads its inputs from
 // registers (loaded straight from global memory: butterfly b takes the
 // elements b + k L/R0, so consecutive threads read consecutive addresses),
 // the middle stages go through LDS (two buffers, one barrier per stage) and
@@ -65,9 +65,10 @@ constexpr void push_pow2(FftPlan& p, int m, int kmax) {
   }
 }
 
-// Radices a stage can have: powers of two up to 16, 3, 5, and the products
-// 6, 9, 10, 12, 15 done in registers by one Cooley-Tukey step (dft_ct).
-constexpr int kMixRadices[11] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+// Radices a stage can have: powers of two up to 16, 3, 5, 7, and the
+// products 6, 9, 10, 12, 15 done in registers by one Cooley-Tukey step
+// (dft_ct).
+constexpr int kMixRadices[12] = {16, 15, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
 
 // The fewest radices (at most `depth`) from kMixRadices whose product is m,
 // largest first, into out; returns their number, or 99 if there are none.
@@ -78,7 +79,7 @@ constexpr int split_radices(int m, int depth, int* out) {
   if (depth == 0 || cap < m) return 99;
   int best = 99;
   int bestv[8] = {};
-  for (int i = 0; i < 11; ++i) {
+  for (int i = 0; i < 12; ++i) {
     const int r = kMixRadices[i];
     if (m % r != 0) continue;
     int sub[8] = {};
@@ -101,7 +102,7 @@ constexpr int split_radices(int m, int depth, int* out) {
 constexpr bool mixed_plan(int L, int E, FftPlan& p) {
   int best_e = 0, best_c = 99, best_key = 1 << 30;
   int best_mid[8] = {};
-  for (int i = 0; i < 11; ++i) {
+  for (int i = 0; i < 12; ++i) {
     const int e = kMixRadices[i];
     if (L % (e * e) != 0) continue;
     int mid[8] = {};
@@ -129,14 +130,15 @@ constexpr bool mixed_plan(int L, int E, FftPlan& p) {
 // (E = 4, 8 or 16; also the largest power-of-two radix used).  The edge
 // radix 2^ke appears first and last so the transform is palindromic.  When
 // L's power-of-two part is too small for edges of E (a line of 2^a 3^b 5^c
-// with 2^a < E^2), the plan of mixed radices with edges closest to E
+// 7^d with 2^a < E^2), the plan of mixed radices with edges closest to E
 // replaces it if it has fewer stages: 1000 = 2 2 5 5 5 2 becomes 10 10 10.
 constexpr FftPlan make_plan(int L, int E) {
   FftPlan p{};
   p.L = L;
   p.n = 0;
   p.ok = L >= 2;
-  int rem = L, n3 = 0, n5 = 0;
+  int rem = L, n3 = 0, n5 = 0, n7 = 0;
+  while (rem % 7 == 0) { ++n7; rem /= 7; }
   while (rem % 5 == 0) { ++n5; rem /= 5; }
   while (rem % 3 == 0) { ++n3; rem /= 3; }
   if (!p.ok || (rem & (rem - 1)) != 0) {
@@ -144,7 +146,7 @@ constexpr FftPlan make_plan(int L, int E) {
     return p;
   }
   const int a2 = ilog2c(rem), kE = ilog2c(E);
-  if (L == 3 || L == 5 || (L == rem && L <= E)) {
+  if (L == 3 || L == 5 || L == 7 || (L == rem && L <= E)) {
     p.r[p.n++] = L;
   } else {
     const int ke = (a2 / 2 < kE) ? a2 / 2 : kE;
@@ -153,13 +155,15 @@ constexpr FftPlan make_plan(int L, int E) {
       push_pow2(p, a2 - 2 * ke, kE);
       for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
       for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
+      for (int i = 0; i < n7; ++i) p.r[p.n++] = 7;
       p.r[p.n++] = 1 << ke;
     } else {
       push_pow2(p, a2, kE);
       for (int i = 0; i < n3; ++i) p.r[p.n++] = 3;
       for (int i = 0; i < n5; ++i) p.r[p.n++] = 5;
+      for (int i = 0; i < n7; ++i) p.r[p.n++] = 7;
     }
-    if (n3 + n5 > 0 && ke < kE) {
+    if (n3 + n5 + n7 > 0 && ke < kE) {
       FftPlan q = p;
       if (mixed_plan(L, E, q) && q.n < p.n) {
         p.n = q.n;
@@ -330,6 +334,39 @@ __device__ __forceinline__ void dft5(float2* x) {
   x[3] = csub(m2, n2);
 }
 
+// X_m = x_0 + sum_k cos(2 pi m k/7) (x_k + x_{7-k})
+//           + DIR i sum_k sin(2 pi m k/7) (x_k - x_{7-k}),  k = 1..3,
+// and X_{7-m} the same with the sine terms negated.
+template <int DIR>
+__device__ __forceinline__ void dft7(float2* x) {
+  constexpr float c1 = 0.62348980185873353053f;   // cos(2 pi/7)
+  constexpr float c2 = -0.22252093395631440429f;  // cos(4 pi/7)
+  constexpr float c3 = -0.90096886790241912624f;  // cos(6 pi/7)
+  constexpr float s1 = 0.78183148246802980871f;   // sin(2 pi/7)
+  constexpr float s2 = 0.97492791218182360702f;   // sin(4 pi/7)
+  constexpr float s3 = 0.43388373911755812048f;   // sin(6 pi/7)
+  const float2 a1 = cadd(x[1], x[6]), b1 = csub(x[1], x[6]);
+  const float2 a2 = cadd(x[2], x[5]), b2 = csub(x[2], x[5]);
+  const float2 a3 = cadd(x[3], x[4]), b3 = csub(x[3], x[4]);
+  const float2 x0 = x[0];
+  const float2 m1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x, x0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y);
+  const float2 m2 = make_float2(x0.x + c2 * a1.x + c3 * a2.x + c1 * a3.x, x0.y + c2 * a1.y + c3 * a2.y + c1 * a3.y);
+  const float2 m3 = make_float2(x0.x + c3 * a1.x + c1 * a2.x + c2 * a3.x, x0.y + c3 * a1.y + c1 * a2.y + c2 * a3.y);
+  const float2 n1 =
+      rot<DIR>(make_float2(s1 * b1.x + s2 * b2.x + s3 * b3.x, s1 * b1.y + s2 * b2.y + s3 * b3.y));
+  const float2 n2 =
+      rot<DIR>(make_float2(s2 * b1.x - s3 * b2.x - s1 * b3.x, s2 * b1.y - s3 * b2.y - s1 * b3.y));
+  const float2 n3 =
+      rot<DIR>(make_float2(s3 * b1.x - s1 * b2.x + s2 * b3.x, s3 * b1.y - s1 * b2.y + s2 * b3.y));
+  x[0] = make_float2(x0.x + a1.x + a2.x + a3.x, x0.y + a1.y + a2.y + a3.y);
+  x[1] = cadd(m1, n1);
+  x[6] = csub(m1, n1);
+  x[2] = cadd(m2, n2);
+  x[5] = csub(m2, n2);
+  x[3] = cadd(m3, n3);
+  x[4] = csub(m3, n3);
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* x);
 
@@ -420,6 +457,7 @@ __device__ __forceinline__ void dft(float2* x) {
   else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
   else if constexpr (R == 5) dft5<DIR>(x);
   else if constexpr (R == 6) dft_ct<2, 3, DIR>(x);
+  else if constexpr (R == 7) dft7<DIR>(x);
   else if constexpr (R == 8) dft8<DIR>(x);
   else if constexpr (R == 9) dft_ct<3, 3, DIR>(x);
   else if constexpr (R == 10) dft_ct<2, 5, DIR>(x);

@@ -1191,6 +1191,7 @@ struct Variant {
 
 template <int L, int E>
 constexpr Variant variant() {
+  static_assert(PlanOf<L, E>::T <= 1024, "a line of this plan needs more than 1024 threads");
   return Variant{E,
                  LengthOps<L, E>::t4,
                  &LengthOps<L, E>::row,

@@ -359,7 +359,7 @@ class Reconstructor:
     def reconstruct_loop(self, Nrepeats, full_output=False, seed=None):
         """Nrepeats reconstructions from independent random starts, advanced
         together on the GPU in one batch (one after another when an axis
-        length has no 2-3-5 plan and runs through Bluestein transforms);
+        length has no Stockham plan and runs through Bluestein transforms);
         reconstruction r equals reconstruct(seed=seed + r)."""
         if not self._ready():
             return None
@@ -369,7 +369,7 @@ class Reconstructor:
         if seed is None:
             seed = int(np.random.randint(0, 2**31 - 1))
         if not all(L == 1 or sp.spx_supported_length(int(L)) for L in shape):
-            # an axis without a 2-3-5 plan runs through Bluestein transforms,
+            # an axis without a Stockham plan runs through Bluestein transforms,
             # which batches do not take: the starts run one after another
             return self._loop_of_singles(n, int(seed), full_output)
         b = sp.Batch(shape, n)

@@ -224,7 +224,7 @@ class Server:
             if r.algorithm not in ALGORITHMS:
                 raise ValueError(f"request {r.rid}: unknown algorithm {r.algorithm!r}")
             if not all(n == 1 or sp.spx_supported_length(int(n)) for n in r.amplitudes.shape):
-                raise ValueError(f"request {r.rid}: shape {r.amplitudes.shape} has an axis without a 2-3-5 plan; "
+                raise ValueError(f"request {r.rid}: shape {r.amplitudes.shape} has an axis without a Stockham plan; "
                                  "batches do not take such shapes (PlainRunner runs them)")
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         t0 = time.perf_counter()

@@ -101,11 +101,13 @@ def test_init_error_codes_without_gpu(spm):
 
 
 def test_supported_lengths(spm):
-    for L in (2, 8, 128, 384, 1024, 4096, 80, 96, 640, 960, 1000, 3, 5, 15, 5120, 6561, 8192):
+    for L in (2, 8, 128, 384, 1024, 4096, 80, 96, 640, 960, 1000, 3, 5, 15, 5120, 6561, 8192,
+              7, 49, 343, 448, 1029, 1400, 2401, 5488, 7168):
         assert spm.spx_supported_length(L) == 1, L
-    # 6250 = 5 10 5 5 5 needs 1250 threads per line; 7, 11, 13, 77 run
-    # through Bluestein transforms, not the Stockham passes
-    for L in (7, 11, 13, 77, 6250, 16384, 1):
+    # 6250 = 5 10 5 5 5 and 7350 = 7 15 10 7 need over 1024 threads per
+    # line; 11, 13, 77 run through Bluestein transforms, not the Stockham
+    # passes
+    for L in (11, 13, 77, 6250, 5250, 7350, 7560, 7840, 16384, 1):
         assert spm.spx_supported_length(L) == 0, L
 
 

@@ -69,9 +69,10 @@ def test_fft_device_entry(sp, o64):
     assert "not supported" in sp.last_error()
 
 
-# lengths without a Stockham plan go through Bluestein's method: 7-smooth,
-# primes, mixed with plan lengths, 2047 (padding 4096) and 4093, 4095
-# (padding 8192, the largest)
+# image transforms of lengths FFTW plans: 7-smooth ones (Stockham passes with
+# radix-7 stages), and through Bluestein's method primes, 121 = 11^2, mixed
+# with plan lengths, 2047 (padding 4096) and 4093, 4095 (padding 8192, the
+# largest)
 @pytest.mark.parametrize("shape", [(1, 97), (1, 2047), (49, 64), (448, 30), (13, 11, 17), (7, 1000), (3, 121, 8),
                                    (30, 7, 2), (1, 4093), (3, 4095), (2053, 4)])
 def test_fft_any_length_matches_oracle(sp, o64, shape):
@@ -110,8 +111,8 @@ def test_fft_any_length_device_entry(sp, o64):
     assert rel(out, o64.fft(x, +1)) < 3e-6
 
 
-def test_blur_any_length_matches_oracle(sp, o32):
-    shape = (49, 50)
+@pytest.mark.parametrize("shape", [(49, 50), (47, 50)])
+def test_blur_any_length_matches_oracle(sp, o32, shape):
     rng = np.random.default_rng(9)
     x = rng.random(shape).astype(np.complex64)
     img = sp.image_from_array(x)

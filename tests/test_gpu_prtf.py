@@ -111,6 +111,6 @@ def test_prtf_rejects_bad_input():
     with pytest.raises(ValueError):
         gp.prtf(np.zeros((4, 8), np.complex64), None)
     assert sp.spx_prtf(None, None, 8, 8, 1, 2, 1, 1, None, None, None, None, None) == -1
-    imgs = np.zeros((2, 7, 11), np.complex64)  # 7 and 11 are not 2^a 3^b 5^c
+    imgs = np.zeros((2, 7, 11), np.complex64)  # 11 has no Stockham plan
     with pytest.raises(RuntimeError):
         gp.prtf(imgs, None)

@@ -57,14 +57,16 @@ def test_loop_equals_single_reconstructions(sp, shape, kind):
     assert not np.array_equal(loop["real_space_final"][0], loop["real_space_final"][1])
 
 
-def test_loop_on_bluestein_shape(sp):
-    """An axis without a 2-3-5 plan (49 = 7^2): the front-end runs through
-    the Bluestein path and reconstruct_loop falls back to single
-    reconstructions (batches do not take such shapes); the schedule test
-    below checks that path against the oracle."""
-    R = _configure((49, 64), "area")
+@pytest.mark.parametrize("shape", [(47, 64), (49, 64)])
+def test_loop_on_odd_lengths(sp, shape):
+    """An axis without a Stockham plan (47): the front-end runs through the
+    Bluestein path and reconstruct_loop falls back to single reconstructions
+    (batches do not take such shapes).  49 = 7^2 has a plan with radix-7
+    stages and runs batched.  The schedule test below checks both paths
+    against the oracle."""
+    R = _configure(shape, "area")
     loop = R.reconstruct_loop(2, full_output=True, seed=900)
-    assert loop["real_space_final"].shape == (2, 49, 64)
+    assert loop["real_space_final"].shape == (2,) + shape
     for r in range(2):
         one = R.reconstruct(seed=900 + r)
         multi = loop["single_outputs"][r]
@@ -114,7 +116,7 @@ def _oracle_reconstruct(o, A, M, S, start, total, phasings, supports, out_img, o
     return imgs, sups, np.array(scores)
 
 
-@pytest.mark.parametrize("shape,center", [((64, 64), False), ((64, 64), True), ((49, 64), False)])
+@pytest.mark.parametrize("shape,center", [((64, 64), False), ((64, 64), True), ((47, 64), False), ((49, 64), False)])
 def test_reconstruct_schedule_matches_oracle(sp, o32, shape, center):
     """reconstruct() against the oracle driven by the reference's loop, from
     the same start model: ER 20, RAAR (beta 0.9 -> 0.8) 10, ER 10; area
@@ -122,8 +124,9 @@ def test_reconstruct_schedule_matches_oracle(sp, o32, shape, center):
     the centre-image update when `center`) for 28 iterations, then static.
     Algorithm and support switches, Bezier schedules and output points must
     line up: scores within 1e-3 relative, supports equal, images within
-    relative L2 1e-3 at every output point.  49 x 64 runs the Bluestein
-    path (unfused iterations, 7^2 rows)."""
+    relative L2 1e-3 at every output point.  47 x 64 runs the Bluestein
+    path (unfused iterations), 49 x 64 the fused passes with radix-7
+    stages."""
     from libspimage_amd import workloads
     from libspimage_amd.reconstructor import Reconstructor
     from oracle.oracle import SUP_AREA, SUP_CENTRE, SUP_STATIC

@@ -117,7 +117,7 @@ def test_prepare_tunes_ahead_without_changing_results(sp):
         ef, fc, model = plain.run_one(q, keep_model=True)
         assert np.array_equal(res[q.rid].model, model), q.rid
     with pytest.raises(ValueError):
-        sp.tune_shape((7, 7), 2)
+        sp.tune_shape((11, 11), 2)
 
 
 def test_served_volumes_of_the_serving_mix(sp):

@@ -1,5 +1,5 @@
 """Request validation of the server (libspimage_amd/serve.py) before any
-device work: shapes whose axes have no 2-3-5 plan (which batches do not
+device work: shapes whose axes have no Stockham plan (which batches do not
 take), iteration counts off the quantum, unknown algorithms."""
 import numpy as np
 import pytest
@@ -14,8 +14,8 @@ def _req(shape, **kw):
 
 
 @pytest.mark.parametrize("shape,kw,msg", [
-    ((49, 64), {}, "2-3-5 plan"),
-    ((64, 64, 7), {}, "2-3-5 plan"),
+    ((47, 64), {}, "Stockham plan"),
+    ((64, 64, 11), {}, "Stockham plan"),
     ((64, 64), {"iterations": 70}, "multiple of 50"),
     ((64, 64), {"algorithm": "dm"}, "unknown algorithm"),
 ])

@@ -65,7 +65,7 @@ def test_fixed_array_transform_rejects_too_large(sp):
 
 
 def test_fixed_array_transform_rejects_bluestein_lengths(sp):
-    # the chirp tables of lengths without a 2-3-5 plan would be allocated
-    a = np.zeros(49 * 64, np.complex64)
-    assert sp.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 64, 49, 1, -1) == -11
+    # the chirp tables of lengths without a Stockham plan would be allocated
+    a = np.zeros(47 * 64, np.complex64)
+    assert sp.spx_fft_noalloc(a.ctypes.data, a.ctypes.data, 64, 47, 1, -1) == -11
     assert "not supported" in sp.last_error()
