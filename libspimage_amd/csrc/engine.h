@@ -216,6 +216,9 @@ class Engine {
   int* alg_rec = nullptr;     // optional per-rec tables
   float* beta_rec = nullptr;
   uint8_t* rec_active = nullptr;
+  uint8_t* active_host = nullptr;  // pinned staging of rec_active's uploads
+  hipEvent_t active_ev = nullptr;  // recorded after the last upload
+  bool active_pending = false;
   std::vector<std::pair<unsigned long long, float*>> templates;  // support templates by key
   bool some_idle = false;
   bool R_valid = false, Rf_valid = false;

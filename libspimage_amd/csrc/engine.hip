@@ -1904,6 +1904,8 @@ Engine::~Engine() {
     if (p) (void)hipFree(p);
   for (auto& e : templates) (void)hipFree(e.second);
   if (red_host) (void)hipHostFree(red_host);
+  if (active_host) (void)hipHostFree(active_host);
+  if (active_ev) (void)hipEventDestroy(active_ev);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (s) (void)hipStreamDestroy(s);
@@ -2006,21 +2008,32 @@ int Engine::set_rec_params(const int* alg, const float* beta) {
   return 0;
 }
 
+// The flags go to the device through a pinned staging buffer, queued on the
+// engine's stream without waiting for it: iterations queued before the call
+// still see the old flags, those queued after it the new ones.  Waiting here
+// would also wait for other streams' work queued ahead on the same hardware
+// queue (milliseconds under a busy server).  A later call first waits for the
+// earlier upload to have read the staging buffer.
 int Engine::set_rec_active(const int* active) {
   bool idle = false;
-  std::vector<uint8_t> h((size_t)nrec, 1);
   if (active)
-    for (int r = 0; r < nrec; ++r) {
-      h[r] = active[r] ? 1 : 0;
+    for (int r = 0; r < nrec; ++r)
       if (!active[r]) idle = true;
-    }
   if (idle) {
     if (!rec_active) {
       drop_graphs();  // a new kernel argument
       SPX_HIP(hipMalloc(&rec_active, (size_t)nrec));
     }
-    SPX_HIP(hipMemcpyAsync(rec_active, h.data(), (size_t)nrec, hipMemcpyHostToDevice, s));
-    SPX_HIP(hipStreamSynchronize(s));
+    if (!active_host) {
+      SPX_HIP(hipHostMalloc((void**)&active_host, (size_t)nrec, hipHostMallocDefault));
+      SPX_HIP(hipEventCreateWithFlags(&active_ev, hipEventDisableTiming));
+    } else if (active_pending) {
+      SPX_HIP(hipEventSynchronize(active_ev));
+    }
+    for (int r = 0; r < nrec; ++r) active_host[r] = active[r] ? 1 : 0;
+    SPX_HIP(hipMemcpyAsync(rec_active, active_host, (size_t)nrec, hipMemcpyHostToDevice, s));
+    SPX_HIP(hipEventRecord(active_ev, s));
+    active_pending = true;
   }
   if (idle != some_idle) drop_graphs();
   some_idle = idle;

@@ -196,6 +196,14 @@ class Server:
         bt.batch.set_active([0 if s is None else 1 for s in bt.slots])
 
     # -- progress ------------------------------------------------------------
+    def _launch(self, bt: _Batch):
+        """Queues the batch's next quantum (and, if a reconstruction finishes
+        with it, the Fourier errors behind it)."""
+        bt.batch.iterate(self.quantum)
+        if any(s is not None and s.done + self.quantum >= s.req.iterations for s in bt.slots):
+            bt.batch.efourier_enqueue()
+        bt.busy = True
+
     def _finish_quantum(self, bt: _Batch, now_fn):
         for s in bt.slots:
             if s is not None:
@@ -246,12 +254,15 @@ class Server:
                         bt.busy = False
                         self._finish_quantum(bt, now)
                         progressed = True
-            # admission into free slots, then new batches for what still waits
+            # admission into free slots, then new batches for what still
+            # waits; a filled batch starts at once, so that the device works
+            # while the host fills the next one
             for shape, q in self.queues.items():
                 if not q:
                     continue
                 for bt in self.classes.get(shape, []):
                     if not bt.busy and self._fill(bt, q, now()):
+                        self._launch(bt)
                         progressed = True
                 while q:
                     # free slots of running batches take requests when their quantum ends
@@ -262,15 +273,13 @@ class Server:
                     if bt is None:
                         break
                     self._fill(bt, q, now())
+                    self._launch(bt)
                     progressed = True
             # launch the next quantum of every idle batch with work
             for blist in self.classes.values():
                 for bt in blist:
                     if not bt.busy and any(s is not None for s in bt.slots):
-                        bt.batch.iterate(self.quantum)
-                        if any(s is not None and s.done + self.quantum >= s.req.iterations for s in bt.slots):
-                            bt.batch.efourier_enqueue()
-                        bt.busy = True
+                        self._launch(bt)
                         progressed = True
             if self.idle_release is not None:
                 self._release_idle(older_than=self.idle_release)
