@@ -38,6 +38,11 @@ spimage_EXPORT int spx_batch_size(const SpxBatch *b);
 /* inputs of reconstruction `rec` (rec < 0: all, arrays of nrec * N) */
 spimage_EXPORT int spx_batch_set_amplitudes(SpxBatch *b, int rec, const float *amplitudes, const int *mask);
 spimage_EXPORT int spx_batch_set_support(SpxBatch *b, int rec, const float *support);
+/* amplitudes, mask and support together: the same state as
+ * spx_batch_set_amplitudes then spx_batch_set_support, without reading the
+ * other input's flags back from the device, and queued without waiting */
+spimage_EXPORT int spx_batch_set_data(SpxBatch *b, int rec, const float *amplitudes, const int *mask,
+                                      const float *support);
 spimage_EXPORT int spx_batch_set_model(SpxBatch *b, int rec, const float *model_complex);
 /* SpModelRandomPhases / SpModelZeroPhases / SpModelRandomValues start for all
  * reconstructions from their amplitudes (reference src/phasing.c:626-679);

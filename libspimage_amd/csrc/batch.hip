@@ -163,6 +163,17 @@ int spx_batch_set_support(SpxBatch* b, int rec, const float* support) {
   return b->e->upload_flags(fl.data(), rec);
 }
 
+int spx_batch_set_data(SpxBatch* b, int rec, const float* amplitudes, const int* mask, const float* support) {
+  if (!check(b, rec) || !amplitudes || !mask || !support) return fail(-1, "bad batch arguments");
+  const long n = rec < 0 ? b->N * b->nrec : b->N, off = rec < 0 ? 0 : rec * b->N;
+  std::vector<uint8_t> fl((size_t)n);
+  for (long i = 0; i < n; ++i) {
+    fl[i] = (uint8_t)((mask[i] ? SpPixelMeasuredAmplitude : 0) | (support[i] != 0.f ? SpPixelInsideSupport : 0));
+    b->amp[off + i] = amplitudes[i];
+  }
+  return b->e->upload_amplitudes(amplitudes, fl.data(), rec);
+}
+
 int spx_batch_set_model(SpxBatch* b, int rec, const float* model) {
   if (!check(b, rec) || !model) return fail(-1, "bad batch arguments");
   return b->e->upload_model((const float2*)model, rec);

@@ -35,6 +35,10 @@ const char* last_error();
   } while (0)
 
 bool device_available();
+// Queues a host-to-device copy on stream s through a ring of pinned host
+// memory and returns without waiting for the stream (arrays above 16 MiB are
+// copied with a wait); the host array may be reused at once.
+int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 // make_shape chooses, once per process and array shape, the plan variant
 // (elements per thread) of each axis by timing the compiled variants on the
 // shape's geometry.  SPX_FFT_E=4|8|16 forces a variant and SPX_NO_TUNE=1
@@ -88,6 +92,7 @@ class Engine {
   hipStream_t s = nullptr;
 
   // ---- host <-> device (rec < 0: all reconstructions, host arrays of NT) ----
+  // uploads return without waiting for the stream (upload_h2d)
   int upload_amplitudes(const float* amp, const uint8_t* flags, int rec = -1);  // also uploads flags
   int upload_flags(const uint8_t* flags, int rec = -1);
   int download_flags(uint8_t* flags, int rec = -1);
@@ -216,9 +221,6 @@ class Engine {
   int* alg_rec = nullptr;     // optional per-rec tables
   float* beta_rec = nullptr;
   uint8_t* rec_active = nullptr;
-  uint8_t* active_host = nullptr;  // pinned staging of rec_active's uploads
-  hipEvent_t active_ev = nullptr;  // recorded after the last upload
-  bool active_pending = false;
   std::vector<std::pair<unsigned long long, float*>> templates;  // support templates by key
   bool some_idle = false;
   bool R_valid = false, Rf_valid = false;

@@ -1904,22 +1904,101 @@ Engine::~Engine() {
     if (p) (void)hipFree(p);
   for (auto& e : templates) (void)hipFree(e.second);
   if (red_host) (void)hipHostFree(red_host);
-  if (active_host) (void)hipHostFree(active_host);
-  if (active_ev) (void)hipEventDestroy(active_ev);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (s) (void)hipStreamDestroy(s);
 }
 
+// ---------------------------------------------------------------------------
+// Host-to-device uploads that return at once.  A host array is copied into a
+// ring of pinned host memory shared by every engine of the process, and its
+// DMA is queued on the engine's stream with an event recorded behind it; a
+// region of the ring is reused once the events of the uploads that held it
+// have completed.  The caller may reuse its array on return, and the upload
+// stays ordered with the stream's kernels.  Waiting on the stream instead
+// would also wait for other streams' work queued ahead on the same hardware
+// queue (milliseconds when many batches are served at once).  Arrays above a
+// quarter of the ring are copied from pageable memory with a wait.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kRingBytes = 64u << 20;
+
+struct UploadRing {
+  std::mutex mu;
+  uint8_t* base = nullptr;
+  size_t head = 0;
+  struct Span {
+    size_t off, len;
+    hipEvent_t ev;
+  };
+  std::vector<Span> live;
+  std::vector<hipEvent_t> spare;
+};
+UploadRing g_ring;
+}  // namespace
+
+int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return 0;
+  if (bytes > kRingBytes / 4) {
+    SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    SPX_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  UploadRing& g = g_ring;
+  std::lock_guard<std::mutex> lock(g.mu);
+  if (!g.base) SPX_HIP(hipHostMalloc((void**)&g.base, kRingBytes, hipHostMallocDefault));
+  const size_t len = (bytes + 255) & ~(size_t)255;
+  size_t off = g.head;
+  if (off + len > kRingBytes) off = 0;
+  // wait for the uploads still reading [off, off + len); drop finished ones
+  for (size_t i = 0; i < g.live.size();) {
+    UploadRing::Span& sp = g.live[i];
+    const bool overlap = sp.off < off + len && off < sp.off + sp.len;
+    bool done = false;
+    if (overlap) {
+      SPX_HIP(hipEventSynchronize(sp.ev));
+      done = true;
+    } else {
+      const hipError_t q = hipEventQuery(sp.ev);
+      if (q != hipSuccess && q != hipErrorNotReady) SPX_HIP(q);
+      done = q == hipSuccess;
+    }
+    if (done) {
+      g.spare.push_back(sp.ev);
+      sp = g.live.back();
+      g.live.pop_back();
+    } else {
+      ++i;
+    }
+  }
+  hipEvent_t ev = nullptr;
+  if (!g.spare.empty()) {
+    ev = g.spare.back();
+    g.spare.pop_back();
+  } else {
+    SPX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  memcpy(g.base + off, src, bytes);
+  const hipError_t e1 = hipMemcpyAsync(dst, g.base + off, bytes, hipMemcpyHostToDevice, s);
+  const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev, s) : e1;
+  if (e2 != hipSuccess) {
+    // nothing queued reads the region: it stays free
+    g.spare.push_back(ev);
+    SPX_HIP(e2);
+  }
+  g.live.push_back({off, len, ev});
+  g.head = off + len;
+  return 0;
+}
+
 // encoded amplitudes: the raw values pass through `absb` (scratch) once
 int Engine::upload_amplitudes(const float* amp, const uint8_t* fl, int rec) {
   const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
-  SPX_HIP(hipMemcpyAsync(absb + off, amp, sizeof(float) * n, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipMemcpyAsync(flags + off, fl, (size_t)n, hipMemcpyHostToDevice, s));
+  if (int rc = upload_h2d(absb + off, amp, sizeof(float) * n, s)) return rc;
+  if (int rc = upload_h2d(flags + off, fl, (size_t)n, s)) return rc;
   hipLaunchKernelGGL(k_encode_amp, ew_grid(n), dim3(256), 0, s, absb + off, flags + off, ampx + off, n, nx, ny,
                      sh.t4 ? 1 : 0);
   SPX_HIP(hipGetLastError());
-  SPX_HIP(hipStreamSynchronize(s));
   ++flags_version;
   ++amp_version;
   return 0;
@@ -1927,8 +2006,7 @@ int Engine::upload_amplitudes(const float* amp, const uint8_t* fl, int rec) {
 
 int Engine::upload_flags(const uint8_t* fl, int rec) {
   const long off = rec < 0 ? 0 : rec * N, n = rec < 0 ? NT : N;
-  SPX_HIP(hipMemcpyAsync(flags + off, fl, (size_t)n, hipMemcpyHostToDevice, s));
-  SPX_HIP(hipStreamSynchronize(s));
+  if (int rc = upload_h2d(flags + off, fl, (size_t)n, s)) return rc;
   ++flags_version;
   return 0;
 }
@@ -1987,33 +2065,30 @@ int Engine::download(const float2* dev, float2* host, int rec) {
 // recapture; a table's first allocation or its removal changes the kernel
 // arguments.  A null table is freed, so iterations go back to the shared
 // algorithm type and beta schedule.
+// The tables and flags below are uploaded in stream order (upload_h2d):
+// iterations queued before a call use the old values, those queued after it
+// the new ones.
 int Engine::set_rec_params(const int* alg, const float* beta) {
   if ((alg && !alg_rec) || (beta && !beta_rec) || (!alg && alg_rec) || (!beta && beta_rec)) drop_graphs();
-  SPX_HIP(hipStreamSynchronize(s));  // no queued iteration may still read a table freed below
+  // queued iterations may still read a table freed below
+  if ((!alg && alg_rec) || (!beta && beta_rec)) SPX_HIP(hipStreamSynchronize(s));
   if (alg) {
     if (!alg_rec) SPX_HIP(hipMalloc(&alg_rec, sizeof(int) * nrec));
-    SPX_HIP(hipMemcpyAsync(alg_rec, alg, sizeof(int) * nrec, hipMemcpyHostToDevice, s));
+    if (int rc = upload_h2d(alg_rec, alg, sizeof(int) * nrec, s)) return rc;
   } else if (alg_rec) {
     SPX_HIP(hipFree(alg_rec));
     alg_rec = nullptr;
   }
   if (beta) {
     if (!beta_rec) SPX_HIP(hipMalloc(&beta_rec, sizeof(float) * nrec));
-    SPX_HIP(hipMemcpyAsync(beta_rec, beta, sizeof(float) * nrec, hipMemcpyHostToDevice, s));
+    if (int rc = upload_h2d(beta_rec, beta, sizeof(float) * nrec, s)) return rc;
   } else if (beta_rec) {
     SPX_HIP(hipFree(beta_rec));
     beta_rec = nullptr;
   }
-  SPX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
-// The flags go to the device through a pinned staging buffer, queued on the
-// engine's stream without waiting for it: iterations queued before the call
-// still see the old flags, those queued after it the new ones.  Waiting here
-// would also wait for other streams' work queued ahead on the same hardware
-// queue (milliseconds under a busy server).  A later call first waits for the
-// earlier upload to have read the staging buffer.
 int Engine::set_rec_active(const int* active) {
   bool idle = false;
   if (active)
@@ -2024,16 +2099,9 @@ int Engine::set_rec_active(const int* active) {
       drop_graphs();  // a new kernel argument
       SPX_HIP(hipMalloc(&rec_active, (size_t)nrec));
     }
-    if (!active_host) {
-      SPX_HIP(hipHostMalloc((void**)&active_host, (size_t)nrec, hipHostMallocDefault));
-      SPX_HIP(hipEventCreateWithFlags(&active_ev, hipEventDisableTiming));
-    } else if (active_pending) {
-      SPX_HIP(hipEventSynchronize(active_ev));
-    }
-    for (int r = 0; r < nrec; ++r) active_host[r] = active[r] ? 1 : 0;
-    SPX_HIP(hipMemcpyAsync(rec_active, active_host, (size_t)nrec, hipMemcpyHostToDevice, s));
-    SPX_HIP(hipEventRecord(active_ev, s));
-    active_pending = true;
+    std::vector<uint8_t> h((size_t)nrec);
+    for (int r = 0; r < nrec; ++r) h[r] = active[r] ? 1 : 0;
+    if (int rc = upload_h2d(rec_active, h.data(), (size_t)nrec, s)) return rc;
   }
   if (idle != some_idle) drop_graphs();
   some_idle = idle;
@@ -2065,7 +2133,7 @@ int Engine::init_model_from_amplitudes(const float* amp_raw, int fl, uint64_t sd
   const int cnt = rec < 0 ? nrec : 1;
   const long off = rec < 0 ? 0 : rec * N, n = cnt * N;
   const Shape& shp = rec < 0 ? sh : one;
-  SPX_HIP(hipMemcpyAsync(absb + off, amp_raw, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  if (int rc = upload_h2d(absb + off, amp_raw, sizeof(float) * n, s)) return rc;
   const float inv_n = (float)(1.0 / (double)N);
   if (fl & 1 /* SpModelRandomPhases */ || fl & 2 /* SpModelZeroPhases */) {
     hipLaunchKernelGGL(k_init_phases, ew_grid(n), dim3(256), 0, s, absb + off, S1 + off, N, n, sd,
@@ -2112,7 +2180,6 @@ int Engine::finish_model_init(int fl, int rec) {
   hipLaunchKernelGGL(k_fill_c64, ew_grid(n), dim3(256), 0, s, g[cur ^ 1] + off, n, make_float2(0.f, 0.f));
   hipLaunchKernelGGL(k_fill_c64, ew_grid(n), dim3(256), 0, s, gp + off, n, make_float2(0.f, 0.f));
   SPX_HIP(hipGetLastError());
-  SPX_HIP(hipStreamSynchronize(s));
   invalidate_R(rec);
   return 0;
 }

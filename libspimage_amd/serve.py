@@ -178,8 +178,7 @@ class Server:
                 continue
             req = q.popleft()
             b = bt.batch
-            b.set_amplitudes(r, req.amplitudes, req.mask)
-            b.set_support(r, req.support)
+            b.set_data(r, req.amplitudes, req.mask, req.support)
             b.init_model_rec(r, sp.SpModelRandomPhases, req.seed)
             bt.algs[r] = ALGORITHMS[req.algorithm]
             bt.betas[r] = req.beta

@@ -283,6 +283,7 @@ _bind("spx_batch_free", None, PBatch)
 _bind("spx_batch_size", C.c_int, PBatch)
 _bind("spx_batch_set_amplitudes", C.c_int, PBatch, C.c_int, PF, PI)
 _bind("spx_batch_set_support", C.c_int, PBatch, C.c_int, PF)
+_bind("spx_batch_set_data", C.c_int, PBatch, C.c_int, PF, PI, PF)
 _bind("spx_batch_set_model", C.c_int, PBatch, C.c_int, PF)
 _bind("spx_batch_init_model", C.c_int, PBatch, C.c_int, C.c_ulonglong)
 _bind("spx_batch_set_algorithm", C.c_int, PBatch, PAlg)
@@ -379,8 +380,7 @@ class Batch:
         a = np.ascontiguousarray(amplitudes, np.float32)
         m = np.ascontiguousarray(mask, np.int32)
         s = np.ascontiguousarray(support, np.float32)
-        self._check(spx_batch_set_amplitudes(self.h, rec, _fptr(a), _iptr(m)), "set_amplitudes")
-        self._check(spx_batch_set_support(self.h, rec, _fptr(s)), "set_support")
+        self._check(spx_batch_set_data(self.h, rec, _fptr(a), _iptr(m), _fptr(s)), "set_data")
 
     def set_model(self, rec, model):
         g = np.ascontiguousarray(model, np.complex64)

@@ -287,3 +287,56 @@ def test_batch_sizes_of_the_batched_workload(sp, o32, shape):
         assert oph.iterate(alg, 1) == 0
         assert rel(one[r], oph.g1) <= 1e-5, r
     b.close()
+
+
+def test_set_data_equals_separate_setters(sp):
+    """spx_batch_set_data builds the flags from the mask and support on the
+    host; the state must equal spx_batch_set_amplitudes then
+    spx_batch_set_support (which read the other input's flags back)."""
+    pats = patterns((96, 80), 3, seed=120)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    sup = sp.sp_support_array_init(sp.sp_support_area_alloc(sp.smap([(0, 2.0)]), sp.smap([(0, 0.1)])), 5)
+    a = make_batch(sp, pats, hio, sup=sup)
+    b = sp.Batch((96, 80), 3)
+    for r, p in enumerate(pats):
+        b.set_amplitudes(r, p.amplitudes, p.mask)
+        b.set_support(r, p.support)
+        b.set_model(r, p.start)
+    b.set_algorithm(hio)
+    b.set_support_schedule(sup)
+    for x in (a, b):
+        x.iterate(12)
+    assert np.array_equal(a.model(), b.model())
+    assert np.array_equal(a.support(), b.support())
+    a.close()
+    b.close()
+
+
+def test_uploads_copy_the_host_arrays_at_once(sp):
+    """The setters queue their uploads without waiting for the device: one
+    host buffer overwritten right after each call must give the same batch
+    as separate arrays (the upload copied it before returning)."""
+    shape = (128, 96)
+    pats = patterns(shape, 6, seed=130)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    ref = make_batch(sp, pats, hio)
+    ref.iterate(4)
+    amp = np.empty(shape, np.float32)
+    msk = np.empty(shape, np.int32)
+    sup = np.empty(shape, np.float32)
+    b = sp.Batch(shape, len(pats))
+    b.set_algorithm(hio)
+    for r, p in enumerate(pats):
+        amp[...], msk[...], sup[...] = p.amplitudes, p.mask, p.support
+        b.set_data(r, amp, msk, sup)
+        amp.fill(np.nan)
+        msk.fill(0)
+        sup.fill(0)
+        b.set_model(r, p.start)
+    b.set_active([1, 0, 1, 1, 0, 1])
+    b.set_active(None)
+    b.iterate(4)
+    assert np.array_equal(b.model(), ref.model())
+    assert np.array_equal(b.support(), ref.support())
+    ref.close()
+    b.close()
