@@ -4,8 +4,10 @@
 // the reference's sp_phaser_iterate scheduler (src/phasing.c:1000-1022).
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <memory>
 #include <vector>
 
 #include "engine.h"
@@ -16,7 +18,9 @@ struct SpxBatch {
   spx::Engine* e = nullptr;
   int nx = 0, ny = 0, nz = 0, nrec = 0;
   long N = 0;
-  std::vector<float> amp;     // host copy of the raw amplitudes (nrec * N)
+  // host copy of the raw amplitudes (nrec * N), zero until set; calloc's
+  // pages stay untouched (and free to map) until a reconstruction is set
+  std::unique_ptr<float, decltype(&free)> amp{nullptr, &free};
   SpPhasingAlgorithm* alg = nullptr;
   SpSupportArray* sup = nullptr;
   bool rec_betas = false;  // per-reconstruction betas replace the schedule
@@ -129,7 +133,12 @@ SpxBatch* spx_batch_create(int nx, int ny, int nz, int nrec) {
   b->nz = nz;
   b->nrec = nrec;
   b->N = (long)nx * ny * nz;
-  b->amp.assign((size_t)b->N * nrec, 0.f);
+  b->amp.reset((float*)calloc((size_t)b->N * nrec, sizeof(float)));
+  if (!b->amp) {
+    fail(-20, "host memory for the amplitudes");
+    spx_batch_free(b);
+    return nullptr;
+  }
   return b;
 }
 
@@ -148,7 +157,7 @@ int spx_batch_set_amplitudes(SpxBatch* b, int rec, const float* amplitudes, cons
   if (int rc = b->e->download_flags(fl.data(), rec)) return rc;
   for (long i = 0; i < n; ++i) {
     fl[i] = (uint8_t)((fl[i] & SpPixelInsideSupport) | (mask[i] ? SpPixelMeasuredAmplitude : 0));
-    b->amp[off + i] = amplitudes[i];
+    b->amp.get()[off + i] = amplitudes[i];
   }
   return b->e->upload_amplitudes(amplitudes, fl.data(), rec);
 }
@@ -169,7 +178,7 @@ int spx_batch_set_data(SpxBatch* b, int rec, const float* amplitudes, const int*
   std::vector<uint8_t> fl((size_t)n);
   for (long i = 0; i < n; ++i) {
     fl[i] = (uint8_t)((mask[i] ? SpPixelMeasuredAmplitude : 0) | (support[i] != 0.f ? SpPixelInsideSupport : 0));
-    b->amp[off + i] = amplitudes[i];
+    b->amp.get()[off + i] = amplitudes[i];
   }
   return b->e->upload_amplitudes(amplitudes, fl.data(), rec);
 }
@@ -181,14 +190,14 @@ int spx_batch_set_model(SpxBatch* b, int rec, const float* model) {
 
 int spx_batch_init_model(SpxBatch* b, int flags, unsigned long long seed) {
   if (!b) return fail(-1, "bad batch arguments");
-  int rc = b->e->init_model_from_amplitudes(b->amp.data(), flags, seed);
+  int rc = b->e->init_model_from_amplitudes(b->amp.get(), flags, seed);
   if (!rc) rc = b->e->finish_model_init(flags);
   return rc;
 }
 
 int spx_batch_init_model_rec(SpxBatch* b, int rec, int flags, unsigned long long seed) {
   if (!b || rec < 0 || rec >= b->nrec) return fail(-1, "bad batch arguments");
-  int rc = b->e->init_model_from_amplitudes(b->amp.data() + (size_t)rec * b->N, flags, seed, rec);
+  int rc = b->e->init_model_from_amplitudes(b->amp.get() + (size_t)rec * b->N, flags, seed, rec);
   if (!rc) rc = b->e->finish_model_init(flags, rec);
   return rc;
 }

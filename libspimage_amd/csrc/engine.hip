@@ -1890,7 +1890,8 @@ int Engine::alloc_all() {
   SPX_HIP(hipMemsetAsync(absb, 0, f, s));
   SPX_HIP(hipMemsetAsync(ctl, 0, sizeof(unsigned) * (size_t)kCtlStride * nrec, s));
   hipLaunchKernelGGL(k_encode_amp, ew_grid(NT), dim3(256), 0, s, absb, flags, ampx, NT, nx, ny, sh.t4 ? 1 : 0);
-  SPX_HIP(hipStreamSynchronize(s));
+  // no wait: everything later on the stream is ordered behind these
+  SPX_HIP(hipGetLastError());
   return 0;
 }
 
