@@ -35,7 +35,10 @@ spimage_EXPORT SpxBatch *spx_batch_create(int nx, int ny, int nz, int nrec);
 spimage_EXPORT void spx_batch_free(SpxBatch *b);
 spimage_EXPORT int spx_batch_size(const SpxBatch *b);
 
-/* inputs of reconstruction `rec` (rec < 0: all, arrays of nrec * N) */
+/* inputs of reconstruction `rec` (rec < 0: all, arrays of nrec * N).  The
+ * setters below copy the host arrays before they return and queue the
+ * uploads on the batch's stream without waiting for it: iterations queued
+ * earlier still see the old values, those queued later the new ones. */
 spimage_EXPORT int spx_batch_set_amplitudes(SpxBatch *b, int rec, const float *amplitudes, const int *mask);
 spimage_EXPORT int spx_batch_set_support(SpxBatch *b, int rec, const float *support);
 /* amplitudes, mask and support together: the same state as
