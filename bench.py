@@ -281,7 +281,7 @@ def disc_bench(probe_dir=None):
     return out
 
 
-def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=2, seed=2000, quantum=25):
+def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_size=None, seed=2000, quantum=25):
     """Many-reconstruction workload: n_patterns seeded disc patterns with sizes
     drawn from {128, 256, 384, 512}^2, n_starts random-phase starts each, HIO
     beta 0.9 for `iters` iterations with an area shrinkwrap every `period`
@@ -292,8 +292,8 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     concurrently, each batch on its own stream, fed by one host loop in
     quanta of `quantum` iterations; the same classes run one after another
     for comparison.  Plain: the reference-style loop, one SpPhaser per
-    reconstruction, timed on `plain_per_size` reconstructions per size class
-    from the same starts and extrapolated to the size class's count.  Inputs
+    reconstruction from the same starts, timed on every reconstruction (or on
+    `plain_per_size` per size class, extrapolated to the class's count).  Inputs
     are uploaded before the clock starts in every form; the clock stops after
     synchronising efourier read-backs."""
     import numpy as np
@@ -326,7 +326,8 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     ef_serial, starts = {}, {}
     for n, pats in sorted(classes.items()):
         b = build(n, pats)
-        starts[n] = [b.model(r) for r in range(plain_per_size)]
+        n_plain = len(pats) * n_starts if plain_per_size is None else min(plain_per_size, len(pats) * n_starts)
+        starts[n] = [b.model(r) for r in range(n_plain)]
         b.synchronize()
         t0 = time.perf_counter()
         b.iterate(iters)
@@ -367,8 +368,9 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     t_plain, per_size, worst = 0.0, {}, 0.0
     for n, pats in sorted(classes.items()):
         nrec = len(pats) * n_starts
+        n_plain = len(starts[n])
         tp, ef_p = 0.0, []
-        for r in range(plain_per_size):
+        for r in range(n_plain):
             p = pats[r // n_starts]
             ph = sp.sp_phaser_alloc()
             if sp.sp_phaser_init(ph, hio, sarr, sp.SpEngineHIP) != 0:
@@ -387,11 +389,11 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
             sp.sp_phaser_free(ph)
             for im in imgs:
                 sp.sp_image_free(im)
-        tp_class = tp / plain_per_size * nrec
+        tp_class = tp / n_plain * nrec
         diff = max(abs(float(ef_serial[n][r]) - e) / max(abs(e), 1e-30) for r, e in enumerate(ef_p))
         worst = max(worst, diff)
         t_plain += tp_class
-        per_size[f"{n}x{n}"] = {"recons": nrec, "plain_s_extrapolated": tp_class,
+        per_size[f"{n}x{n}"] = {"recons": nrec, "plain_recons_timed": n_plain, "plain_s": tp_class,
                                 "mean_efourier": float(np.mean(ef_serial[n]))}
     total = n_patterns * n_starts
     return {"workload": f"{n_patterns} patterns x {n_starts} random starts, sizes {{128,256,384,512}}^2, "
@@ -401,9 +403,10 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
                             f"(quanta of {quantum} iterations)",
             "serial_classes_seconds": t_serial, "serial_classes_recons_per_s": total / t_serial,
             "concurrent_equals_serial": same,
-            "plain_seconds_extrapolated": t_plain, "plain_recons_per_s": total / t_plain,
+            "plain_seconds": t_plain, "plain_recons_per_s": total / t_plain,
             "speedup": t_plain / t_conc, "speedup_serial_classes": t_plain / t_serial,
-            "plain_sample": f"{plain_per_size} reconstructions per size class",
+            "plain_sample": ("every reconstruction" if plain_per_size is None else
+                             f"{plain_per_size} reconstructions per size class, extrapolated"),
             "efourier_max_rel_diff_batched_vs_plain": worst, "per_size": per_size,
             "hio_iterations_device": device,
             "whole_run_frac_survey_bytes": survey_bytes / t_conc / PEAK_HBM,
