@@ -1928,12 +1928,17 @@ struct UploadRing {
   std::mutex mu;
   uint8_t* base = nullptr;
   size_t head = 0;
+  // an event is recorded on streams of the device it was created on
+  struct Ev {
+    hipEvent_t ev;
+    int dev;
+  };
   struct Span {
     size_t off, len;
-    hipEvent_t ev;
+    Ev ev;
   };
   std::vector<Span> live;
-  std::vector<hipEvent_t> spare;
+  std::vector<Ev> spare;
 };
 UploadRing g_ring;
 }  // namespace
@@ -1946,6 +1951,8 @@ int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     return 0;
   }
   UploadRing& g = g_ring;
+  int dev = 0;
+  SPX_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g.mu);
   if (!g.base) SPX_HIP(hipHostMalloc((void**)&g.base, kRingBytes, hipHostMallocDefault));
   const size_t len = (bytes + 255) & ~(size_t)255;
@@ -1957,10 +1964,10 @@ int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     const bool overlap = sp.off < off + len && off < sp.off + sp.len;
     bool done = false;
     if (overlap) {
-      SPX_HIP(hipEventSynchronize(sp.ev));
+      SPX_HIP(hipEventSynchronize(sp.ev.ev));
       done = true;
     } else {
-      const hipError_t q = hipEventQuery(sp.ev);
+      const hipError_t q = hipEventQuery(sp.ev.ev);
       if (q != hipSuccess && q != hipErrorNotReady) SPX_HIP(q);
       done = q == hipSuccess;
     }
@@ -1972,16 +1979,18 @@ int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
       ++i;
     }
   }
-  hipEvent_t ev = nullptr;
-  if (!g.spare.empty()) {
-    ev = g.spare.back();
-    g.spare.pop_back();
-  } else {
-    SPX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  }
+  UploadRing::Ev ev{nullptr, dev};
+  for (size_t i = 0; i < g.spare.size(); ++i)
+    if (g.spare[i].dev == dev) {
+      ev = g.spare[i];
+      g.spare[i] = g.spare.back();
+      g.spare.pop_back();
+      break;
+    }
+  if (!ev.ev) SPX_HIP(hipEventCreateWithFlags(&ev.ev, hipEventDisableTiming));
   memcpy(g.base + off, src, bytes);
   const hipError_t e1 = hipMemcpyAsync(dst, g.base + off, bytes, hipMemcpyHostToDevice, s);
-  const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev, s) : e1;
+  const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev.ev, s) : e1;
   if (e2 != hipSuccess) {
     // nothing queued reads the region: it stays free
     g.spare.push_back(ev);
