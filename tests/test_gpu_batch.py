@@ -340,3 +340,27 @@ def test_uploads_copy_the_host_arrays_at_once(sp):
     assert np.array_equal(b.support(), ref.support())
     ref.close()
     b.close()
+
+
+def test_uploads_through_a_wrapped_ring(sp):
+    """Twelve 1024^2 slots upload about 108 MB of amplitudes and flags, more
+    than the 64 MiB pinned ring the uploads go through, so the ring wraps
+    while earlier uploads may still be queued.  The batch must equal one
+    filled with a device synchronisation after every slot."""
+    shape = (1024, 1024)
+    pats = patterns(shape, 3, seed=140)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    out = []
+    for sync in (False, True):
+        b = sp.Batch(shape, 12)
+        b.set_algorithm(hio)
+        for r in range(12):
+            p = pats[r % 3]
+            b.set_data(r, p.amplitudes, p.mask, p.support)
+            b.init_model_rec(r, sp.SpModelRandomPhases, 1000 + r)
+            if sync:
+                b.synchronize()
+        b.iterate(2)
+        out.append(b.model())
+        b.close()
+    assert np.array_equal(out[0], out[1])
