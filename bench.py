@@ -310,7 +310,7 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     keep += [bmap, amap]
     sarr = sp.sp_support_array_init(sp.sp_support_area_alloc(bmap, amap), period)
 
-    def build(n, pats):
+    def build(n, pats, graphs=False):
         b = sp.Batch((n, n), len(pats) * n_starts)
         for j, p in enumerate(pats):
             for s_ in range(n_starts):
@@ -318,8 +318,19 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
         b.init_model(sp.SpModelRandomPhases, seed + n)
         b.set_algorithm(hio)
         b.set_support_schedule(sarr)
-        b.use_graphs(False)
+        b.use_graphs(graphs)
         return b
+
+    def concurrent(graphs):
+        bs = {n: build(n, pats, graphs) for n, pats in sorted(classes.items())}
+        for b in bs.values():
+            b.synchronize()
+        t0 = time.perf_counter()
+        for q in range(0, iters, quantum):
+            for b in bs.values():
+                b.iterate(min(quantum, iters - q))
+        ef = {n: b.efourier()[0] for n, b in bs.items()}
+        return bs, ef, time.perf_counter() - t0
 
     # serial classes
     t_serial = 0.0
@@ -334,16 +345,13 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
         ef_serial[n], _ = b.efourier()
         t_serial += time.perf_counter() - t0
         b.close()
-    # concurrent classes
-    bs = {n: build(n, pats) for n, pats in sorted(classes.items())}
-    for b in bs.values():
-        b.synchronize()
-    t0 = time.perf_counter()
-    for q in range(0, iters, quantum):
-        for b in bs.values():
-            b.iterate(min(quantum, iters - q))
-    ef_conc = {n: b.efourier()[0] for n, b in bs.items()}
-    t_conc = time.perf_counter() - t0
+    # concurrent classes replaying captured HIP graphs (one capture per
+    # class, one graph launch per iteration between support updates)
+    bg, ef_graph, t_graph = concurrent(True)
+    for b in bg.values():
+        b.close()
+    # concurrent classes, eager launches
+    bs, ef_conc, t_conc = concurrent(False)
     # device time of each class's plain HIO iteration (no support update),
     # against the survey's 78 N bytes and against the bytes rocprof counted
     # for that class's two passes
@@ -364,6 +372,7 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
     for b in bs.values():
         b.close()
     same = all(np.array_equal(ef_conc[n], ef_serial[n]) for n in classes)
+    same_graph = all(np.array_equal(ef_graph[n], ef_conc[n]) for n in classes)
     # plain loop over the first reconstructions of each class
     t_plain, per_size, worst = 0.0, {}, 0.0
     for n, pats in sorted(classes.items()):
@@ -403,6 +412,8 @@ def batched_bench(n_patterns=128, n_starts=8, iters=500, period=25, plain_per_si
                             f"(quanta of {quantum} iterations)",
             "serial_classes_seconds": t_serial, "serial_classes_recons_per_s": total / t_serial,
             "concurrent_equals_serial": same,
+            "graph_replay_seconds": t_graph, "graph_replay_recons_per_s": total / t_graph,
+            "graph_replay_equals_eager": same_graph,
             "plain_seconds": t_plain, "plain_recons_per_s": total / t_plain,
             "speedup": t_plain / t_conc, "speedup_serial_classes": t_plain / t_serial,
             "plain_sample": ("every reconstruction" if plain_per_size is None else
