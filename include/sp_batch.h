@@ -62,10 +62,13 @@ spimage_EXPORT int spx_batch_set_algorithm(SpxBatch *b, SpPhasingAlgorithm *alg)
  * and beta schedule; NULL restores them) */
 spimage_EXPORT int spx_batch_set_rec_algorithms(SpxBatch *b, const int *alg_types, const float *betas);
 spimage_EXPORT int spx_batch_set_support_schedule(SpxBatch *b, SpSupportArray *sup);
-/* Which reconstructions iterations advance: nrec entries, 0 = idle (its
- * iterate, support and Fourier-space state stay as they are, and its
- * work-groups exit at once); NULL = all active.  Support updates and
- * metrics still cover every reconstruction. */
+/* Which reconstructions iterations advance: nrec entries, 1 = active,
+ * 0 = idle (its iterate, support and Fourier-space state stay as they are;
+ * carrying the iterate over costs a copy of it per iteration), 2 = vacant
+ * (nothing is kept: its work-groups exit at once, and the slot must be set
+ * again, e.g. with spx_batch_set_data and spx_batch_init_model_rec, before
+ * it is made active); NULL = all active.  Support updates and metrics still
+ * cover every reconstruction. */
 spimage_EXPORT int spx_batch_set_active(SpxBatch *b, const int *active);
 /* replay each iteration from a captured HIP graph (1) or launch kernels (0) */
 spimage_EXPORT int spx_batch_use_graphs(SpxBatch *b, int on);

@@ -2099,18 +2099,20 @@ int Engine::set_rec_params(const int* alg, const float* beta) {
   return 0;
 }
 
+// active[r]: 0 idle, 2 vacant, any other value active
 int Engine::set_rec_active(const int* active) {
   bool idle = false;
+  std::vector<uint8_t> h((size_t)nrec, kSlotActive);
   if (active)
-    for (int r = 0; r < nrec; ++r)
-      if (!active[r]) idle = true;
+    for (int r = 0; r < nrec; ++r) {
+      h[r] = active[r] == 2 ? kSlotVacant : active[r] ? kSlotActive : kSlotIdle;
+      if (h[r] != kSlotActive) idle = true;
+    }
   if (idle) {
     if (!rec_active) {
       drop_graphs();  // a new kernel argument
       SPX_HIP(hipMalloc(&rec_active, (size_t)nrec));
     }
-    std::vector<uint8_t> h((size_t)nrec);
-    for (int r = 0; r < nrec; ++r) h[r] = active[r] ? 1 : 0;
     if (int rc = upload_h2d(rec_active, h.data(), (size_t)nrec, s)) return rc;
   }
   if (idle != some_idle) drop_graphs();

@@ -137,6 +137,12 @@ struct RowArgs {
   int L;
 };
 
+// States of a batch slot in the per-reconstruction activity table
+// (spx_batch_set_active): active slots iterate; idle ones keep their state
+// unchanged across iterations; vacant ones keep nothing (their passes do
+// no work) until the slot is set again.
+constexpr uint8_t kSlotIdle = 0, kSlotActive = 1, kSlotVacant = 2;
+
 struct UpdateArgs {
   float2* W;               // in: column-pass output C; out: FFT_x of the new iterate
   float2* Wf;              // diff map: same for f1
@@ -147,7 +153,7 @@ struct UpdateArgs {
   const float2* tw;
   const float* beta_rec;   // per-reconstruction beta, or null for `beta`
   const int* alg_rec;      // per-reconstruction algorithm, or null for `alg`
-  const uint8_t* rec_active;  // per-reconstruction 0 = idle slot (left unchanged), or null
+  const uint8_t* rec_active;  // per-reconstruction slot state (kSlot*), or null for all active
   int nlines;
   int lines_per_rec;
   int alg;
@@ -188,8 +194,9 @@ struct ColArgs {
   int nx, ny, nz;
   float sigma;
   int nt;                    // line data loaded (bit 0) and stored (bit 1) with the non-temporal hint
-  // idle reconstructions (0 = skip the work-group), or null; outer index /
-  // outer_per_rec is the reconstruction
+  // per-reconstruction slot state (kSlot*; the work-groups of idle and
+  // vacant slots return at once), or null; outer index / outer_per_rec is
+  // the reconstruction
   const uint8_t* rec_active;
   int outer_per_rec;
 };
@@ -447,24 +454,27 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_update(UpdateArg
   bool act = line < a.nlines;
   bool idle_line = false;
   if (a.rec_active) {
-    // idle reconstructions keep their state: W stays in place and the
-    // iterate is carried over to the other ping-pong buffer; a work-group
-    // whose lines all belong to idle ones does only that copy and leaves
-    // before its first barrier
+    // kSlotIdle reconstructions keep their state: W stays in place and the
+    // iterate is carried over to the other ping-pong buffer; kSlotVacant
+    // ones keep nothing, so their lines are skipped.  A work-group without
+    // an active line does only those copies and leaves before its first
+    // barrier
     bool any = false;
     for (int k = 0; k < M; ++k) {
       const int l = blk * M + k;
-      if (l < a.nlines && a.rec_active[l / a.lines_per_rec]) any = true;
+      if (l < a.nlines && a.rec_active[l / a.lines_per_rec] == kSlotActive) any = true;
     }
     if (!any) {
       for (int i = tid; i < M * L; i += RG::NT) {
         const int l = blk * M + i / L;
-        if (l < a.nlines) a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
+        if (l < a.nlines && a.rec_active[l / a.lines_per_rec] == kSlotIdle)
+          a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
       }
       return;
     }
-    idle_line = act && !a.rec_active[line / a.lines_per_rec];
-    act = act && !idle_line;
+    const uint8_t st = act ? a.rec_active[line / a.lines_per_rec] : kSlotActive;
+    idle_line = act && st == kSlotIdle;
+    act = act && st == kSlotActive;
   }
   const View va{shA, g}, vb{shB, g};
   // idle lanes of a partial last work-group address the last line (loads
@@ -627,22 +637,24 @@ __global__ __launch_bounds__((RowGeo<L, E, T4>::NT)) void k_row_plain(UpdateArgs
   bool idle_line = false;
   if constexpr (TAB) {
     if (a.rec_active) {
-      // idle reconstructions keep their state: W stays in place and the
-      // iterate is carried over to the other ping-pong buffer
+      // kSlotIdle reconstructions keep their state (the iterate is carried
+      // over to the other ping-pong buffer); kSlotVacant ones are skipped
       bool any = false;
       for (int k = 0; k < M; ++k) {
         const int l = blk * M + k;
-        if (l < a.nlines && a.rec_active[l / a.lines_per_rec]) any = true;
+        if (l < a.nlines && a.rec_active[l / a.lines_per_rec] == kSlotActive) any = true;
       }
       if (!any) {
         for (int i = tid; i < M * L; i += RG::NT) {
           const int l = blk * M + i / L;
-          if (l < a.nlines) a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
+          if (l < a.nlines && a.rec_active[l / a.lines_per_rec] == kSlotIdle)
+            a.gnew[(long)l * L + i % L] = a.gcur[(long)l * L + i % L];
         }
         return;
       }
-      idle_line = act && !a.rec_active[line / a.lines_per_rec];
-      act = act && !idle_line;
+      const uint8_t st = act ? a.rec_active[line / a.lines_per_rec] : kSlotActive;
+      idle_line = act && st == kSlotIdle;
+      act = act && st == kSlotActive;
     }
   }
   const View va{shA, g}, vb{shB, g};
@@ -800,7 +812,7 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col(ColArgs a) {
   const int ngrp = (a.geo.ncontig + C - 1) / C;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int outer = blk / ngrp;
-  if (MODE != kColEFourier && a.rec_active && !a.rec_active[outer / a.outer_per_rec]) return;
+  if (MODE != kColEFourier && a.rec_active && a.rec_active[outer / a.outer_per_rec] != kSlotActive) return;
   const int x = (blk - outer * ngrp) * C + c;
   const bool act = x < a.geo.ncontig;
   // idle lanes address the last line of the group (in bounds, no predicate)
@@ -946,7 +958,7 @@ __global__ __launch_bounds__((ColShape<L, E, WIDE>::NT)) void k_col_plain(ColArg
   const int ngrp = (a.geo.ncontig + C - 1) / C;
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const int outer = blk / ngrp;
-  if (a.rec_active && !a.rec_active[outer / a.outer_per_rec]) return;
+  if (a.rec_active && a.rec_active[outer / a.outer_per_rec] != kSlotActive) return;
   const int x = (blk - outer * ngrp) * C + c;
   const bool act = x < a.geo.ncontig;
   const int xc = act ? x : a.geo.ncontig - 1;
@@ -1004,7 +1016,7 @@ __global__ __launch_bounds__((RowShape<L, E>::NT)) void k_line_project(ColArgs a
   const int nlines = a.geo.ncontig * a.geo.nouter;
   const int line = blk * M + g;
   const int lc = line < nlines ? line : nlines - 1;
-  const bool act = line < nlines && !(a.rec_active && !a.rec_active[lc / a.geo.ncontig]);
+  const bool act = line < nlines && !(a.rec_active && a.rec_active[lc / a.geo.ncontig] != kSlotActive);
   const long base = (long)lc * L;
   const View va{shA, g}, vb{shB, g};
   float2 v[NR];

@@ -24,8 +24,9 @@ Scheduling:
   estimated footprint (spx_batch_estimate_bytes) fits into the free device
   memory minus a reserve; otherwise idle batches of other classes are
   released first and the requests wait;
-* idle slots cost nothing: the batch's activity mask (spx_batch_set_active)
-  makes the pass kernels skip the work-groups of slots without a request;
+* free slots cost nothing: the batch's activity table (spx_batch_set_active)
+  marks them vacant, so the pass kernels skip their work-groups without
+  carrying a stale iterate over;
 * every batch owns a HIP stream, so quanta of different classes overlap on
   the device; the host loop only polls (spx_batch_query) and never waits
   for one class while another could be fed: the Fourier errors of the
@@ -191,8 +192,10 @@ class Server:
 
     @staticmethod
     def _sync_active(bt: _Batch):
-        # idle slots are skipped by the pass kernels
-        bt.batch.set_active([0 if s is None else 1 for s in bt.slots])
+        # free slots are vacant: the pass kernels skip them without carrying
+        # their stale iterate over (a request that takes the slot sets it
+        # afresh)
+        bt.batch.set_active([2 if s is None else 1 for s in bt.slots])
 
     # -- progress ------------------------------------------------------------
     def _launch(self, bt: _Batch):

@@ -479,7 +479,9 @@ class Batch:
         self._check(spx_batch_synchronize(self.h), "synchronize")
 
     def set_active(self, active=None):
-        """nrec flags (0 = idle slot, left unchanged by iterations), or None for all."""
+        """nrec states: 1 active, 0 idle (left unchanged by iterations), 2
+        vacant (nothing kept until the slot is set again); None for all
+        active."""
         a = None if active is None else np.ascontiguousarray(active, np.int32)
         self._check(spx_batch_set_active(self.h, None if a is None else _iptr(a)), "set_active")
 

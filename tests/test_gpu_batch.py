@@ -173,6 +173,36 @@ def test_idle_reconstructions_are_left_unchanged(sp, shape):
     b.close()
 
 
+@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (16, 16, 16)])
+def test_vacant_slots_do_no_work_and_can_be_refilled(sp, shape):
+    """spx_batch_set_active state 2 (vacant): the active slots advance exactly
+    as they do alone; a vacant slot set again (model and data) and made
+    active runs like a fresh reconstruction (row-major, tiled and 3-D)."""
+    pats = patterns(shape, 3, seed=93)
+    hio = sp.sp_phasing_hio_alloc(sp.smap([(0, 0.9)]), 0)
+    b = make_batch(sp, pats, hio)
+    b.set_active([1, 2, 1])
+    b.iterate(7)
+    got = b.model()
+    for r in (0, 2):
+        m, s, e = run_single(sp, pats[r], "hio", 7)
+        assert np.array_equal(got[r], m), r
+    # the vacant slot did no work: after an odd number of iterations the
+    # current ping-pong buffer is the one its start was not written to,
+    # still as the batch was created (zero), not an iterate
+    assert not np.any(got[1])
+    # slot 1 takes a reconstruction afresh
+    b.set_data(1, pats[1].amplitudes, pats[1].mask, pats[1].support)
+    b.set_model(1, pats[1].start)
+    b.set_active(None)
+    b.iterate(3)
+    m, s, e = run_single(sp, pats[1], "hio", 3)
+    assert np.array_equal(b.model(1), m)
+    m, s, e = run_single(sp, pats[0], "hio", 10)
+    assert np.array_equal(b.model(0), m)
+    b.close()
+
+
 @pytest.mark.parametrize("shape", [(64, 64), (16, 16, 16)])
 def test_queued_efourier_equals_blocking(sp, shape):
     """spx_batch_efourier_enqueue / _fetch: the errors queued behind the
