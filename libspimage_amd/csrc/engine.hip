@@ -1992,7 +1992,9 @@ int upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   const hipError_t e1 = hipMemcpyAsync(dst, g.base + off, bytes, hipMemcpyHostToDevice, s);
   const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev.ev, s) : e1;
   if (e2 != hipSuccess) {
-    // nothing queued reads the region: it stays free
+    // the region stays free: nothing was queued, or the queued copy is
+    // waited for here since no event guards it
+    if (e1 == hipSuccess) (void)hipStreamSynchronize(s);
     g.spare.push_back(ev);
     SPX_HIP(e2);
   }
