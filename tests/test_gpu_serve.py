@@ -62,6 +62,29 @@ def test_served_equals_alone(sp, realtime):
         assert got.finished >= got.admitted >= (got.arrival if realtime else 0.0)
 
 
+# lengths with radix-7 stages: 2-D 56 x 98 and 49 x 64, 3-D 14 x 21 x 28
+MIX7 = [((56, 98), "hio", 100), ((49, 64), "raar", 50), ((56, 98), "er", 150), ((14, 21, 28), "hio", 100),
+        ((49, 64), "hio", 100), ((56, 98), "raar", 50), ((14, 21, 28), "er", 50), ((49, 64), "er", 150),
+        ((56, 98), "hio", 200)]
+
+
+def test_served_radix7_shapes_equal_alone(sp):
+    """7-smooth shapes take batches and the server like the 2-3-5 ones; freed
+    slots (vacant) are reused by later requests of the class."""
+    from libspimage_amd import serve
+    reqs = _requests(MIX7, seed0=700)
+    srv = serve.Server(quantum=50, max_pixels_per_batch=2 * 56 * 98, reserve_bytes=0, keep_models=True)
+    res = srv.run(reqs, realtime=True)
+    srv.close()
+    assert sorted(r.rid for r in res) == list(range(len(reqs)))
+    plain = serve.PlainRunner()
+    by_id = {r.rid: r for r in res}
+    for req in reqs:
+        ef, fc, model = plain.run_one(req, keep_model=True)
+        assert np.array_equal(by_id[req.rid].model, model), (req.rid, req.amplitudes.shape)
+        assert abs(by_id[req.rid].efourier - ef) <= 1e-6 * abs(ef), req.rid
+
+
 def test_batch_slot_start_equals_single_seeded_start(sp):
     """spx_batch_init_model_rec(rec, seed) == spx_phaser_set_seed(seed) +
     sp_phaser_init_model(NULL, SpModelRandomPhases)."""
