@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import ctypes as C
 import logging
+import os
 
 import numpy as np
 
@@ -63,6 +64,33 @@ def _shift_into(dst, src, axes):
         for ax, (ds, ss) in zip(axes, combo):
             d[ax], s_[ax] = ds, ss
         dst[tuple(d)] = src[tuple(s_)]
+
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1), thread_name_prefix="spx-outputs")
+    return _POOL
+
+
+def _shift_into_recs(dst, src, axes):
+    """_shift_into for arrays of many reconstructions (axis 0), split across
+    threads by reconstruction: numpy releases the interpreter lock in its
+    copy and conversion loops, so the outputs of a batch are assembled on
+    several cores."""
+    n = src.shape[0]
+    k = min(n, _pool()._max_workers)
+    if k < 2 or src.size < (1 << 20):
+        _shift_into(dst, src, axes)
+        return
+    bounds = [n * i // k for i in range(k + 1)]
+    futs = [_pool().submit(_shift_into, dst[a:b], src[a:b], axes) for a, b in zip(bounds, bounds[1:])]
+    for f in futs:
+        f.result()
+
 
 class Reconstructor:
     def __init__(self, use_gpu=True):
@@ -397,10 +425,10 @@ class Reconstructor:
                     b.set_support_schedule(supports[cs][1])
                 if oi:
                     img = b.model_before_projection(staged=True) if it > 0 else b.model()
-                    _shift_into(real_space[:, k_img], img, axes)
+                    _shift_into_recs(real_space[:, k_img], img, axes)
                     msk = b.support() != 0 if it > 0 else np.broadcast_to(S != 0, (n,) + shape)
-                    _shift_into(support[:, k_img], msk, axes)
-                    _shift_into(fourier_space[:, k_img], b.fmodel(staged=True), axes)
+                    _shift_into_recs(support[:, k_img], msk, axes)
+                    _shift_into_recs(fourier_space[:, k_img], b.fmodel(staged=True), axes)
                     k_img += 1
                 if os_:
                     ef, _ = b.efourier()
