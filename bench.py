@@ -63,6 +63,12 @@ def parse():
                     help="only the many-reconstruction batched workload (a short program for profiler runs)")
     ap.add_argument("--only-volume", action="store_true",
                     help="only the 256^3 reconstruction (a short program for profiler runs)")
+    ap.add_argument("--only-serving-capacity", action="store_true",
+                    help="only the all-queued run of the served workload (a short program for kernel traces)")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="summarise a rocprofv3 kernel trace (busy and idle time, overlap, top kernels); no GPU")
+    ap.add_argument("--trace-window-s", type=float, default=None,
+                    help="with --trace-summary: only the kernels of the trace's last SECONDS")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only launch 20 plain 64 MiB device-to-device copies (counter calibration)")
     return ap.parse_args()
@@ -616,15 +622,12 @@ def _serving_pattern(args):
     return workloads.serving_pattern(shape, seed)
 
 
-def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
-    """Served workload: a Poisson trace at `rate` requests/s for `seconds`
-    (85 % 2-D n x n with n FFT-smooth in [128, 1024], 15 % 3-D n^3 with n in
-    {64, 80, 96, 128}; ER / HIO (beta 0.9) / RAAR (beta 0.87); 100-1000
-    iterations) through the continuous-batching server, in real time and
-    with every request queued at once (capacity), and through the plain
-    reference-style loop (one SpPhaser per request, in arrival order)."""
+def _serving_setup(seconds, rate, seed, workers):
+    """The served workload's trace and patterns, and a warm server: the
+    tuner runs for every shape at every batch size the server may create
+    (Server.prepare), then every shape goes once through both paths
+    (twiddle tables, first launches).  Returns (trace, requests, plain)."""
     from concurrent.futures import ThreadPoolExecutor
-    import numpy as np
     from libspimage_amd import spimage as sp
     from libspimage_amd import serve, workloads
     sizes = [n for n in workloads.smooth_sizes(128, 1024) if sp.spx_supported_length(n)]
@@ -640,9 +643,6 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
                               beta=0.9 if alg == "hio" else 0.87, arrival=t if realtime else 0.0)
                 for k, (t, s, alg, its, sd) in enumerate(trace) if keep is None or keep[k]]
 
-    # warm-up: the tuner runs for every shape at every batch size the server
-    # may create (Server.prepare), then every shape goes once through both
-    # paths (twiddle tables, first launches)
     srv = serve.Server()
     srv.prepare(shapes)
     warm = [serve.Request(-1 - i, pats[s].amplitudes, pats[s].mask, pats[s].support, "hio", 50, 1)
@@ -652,6 +652,73 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
         plain.run_one(r)
     srv.run(warm, realtime=False)
     srv.close()
+    return trace, requests, plain
+
+
+def serving_capacity(seconds=3.0, rate=200.0, seed=4000, workers=8):
+    """Only the all-queued run of the served workload, after the same
+    warm-up (a short program for kernel traces: its kernels are the last
+    `seconds` of the trace)."""
+    from libspimage_amd import serve
+    trace, requests, _ = _serving_setup(seconds, rate, seed, workers)
+    srv = serve.Server()
+    t0 = time.perf_counter()
+    srv.run(requests(False), realtime=False)
+    t = time.perf_counter() - t0
+    srv.close()
+    return {"requests": len(trace), "seconds": t, "recons_per_s": len(trace) / t}
+
+
+def trace_summary(path, window_s=None):
+    """Summary of a rocprofv3 kernel trace (CSV): over the kernels that start
+    in the last `window_s` seconds before the trace's last kernel ends (all
+    of them without a window), the time with no kernel, one, two, ...
+    running, the union of their busy time, and the kernels with the most
+    time (template arguments kept, call arguments dropped)."""
+    import csv
+    import re
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), re.sub(r"\(.*", "", r["Kernel_Name"])))
+    if not rows:
+        return {"kernels": 0}
+    rows.sort()
+    end = max(e for _, e, _ in rows)
+    if window_s is not None:
+        lo = end - int(window_s * 1e9) - 20_000_000  # 20 ms of slack before the window
+        rows = [r for r in rows if r[0] >= lo]
+    events = sorted([(s, 1) for s, _, _ in rows] + [(e, -1) for _, e, _ in rows])
+    busy_at = {}
+    cur, last = 0, events[0][0]
+    for t, d in events:
+        busy_at[cur] = busy_at.get(cur, 0) + (t - last)
+        cur += d
+        last = t
+    per = {}
+    for s, e, n in rows:
+        c, tot = per.get(n, (0, 0))
+        per[n] = (c + 1, tot + e - s)
+    top = sorted(per.items(), key=lambda kv: -kv[1][1])[:15]
+    span = max(e for _, e, _ in rows) - rows[0][0]
+    return {"kernels": len(rows), "span_s": span / 1e9,
+            "busy_union_s": (span - busy_at.get(0, 0)) / 1e9, "idle_s": busy_at.get(0, 0) / 1e9,
+            "seconds_with_k_kernels_running": {str(k): v / 1e9 for k, v in sorted(busy_at.items())},
+            "sum_of_kernel_durations_s": sum(e - s for s, e, _ in rows) / 1e9,
+            "top_kernels": [{"kernel": n, "calls": c, "total_s": t / 1e9, "mean_us": t / c / 1e3}
+                            for n, (c, t) in top]}
+
+
+def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
+    """Served workload: a Poisson trace at `rate` requests/s for `seconds`
+    (85 % 2-D n x n with n FFT-smooth in [128, 1024], 15 % 3-D n^3 with n in
+    {64, 80, 96, 128}; ER / HIO (beta 0.9) / RAAR (beta 0.87); 100-1000
+    iterations) through the continuous-batching server, in real time and
+    with every request queued at once (capacity), and through the plain
+    reference-style loop (one SpPhaser per request, in arrival order)."""
+    import numpy as np
+    from libspimage_amd import serve
+    trace, requests, plain = _serving_setup(seconds, rate, seed, workers)
 
     srv = serve.Server()
     t0 = time.perf_counter()
@@ -928,6 +995,9 @@ def copy_calibration():
 
 def main():
     args = parse()
+    if args.trace_summary:
+        print(json.dumps(trace_summary(args.trace_summary, args.trace_window_s)))
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -945,6 +1015,9 @@ def main():
         return
     if args.only_batched:
         print(json.dumps({"batched": batched_bench()}))
+        return
+    if args.only_serving_capacity:
+        print(json.dumps({"serving_capacity": serving_capacity()}))
         return
     if args.only_headline:
         args.no_batched = args.no_serving = args.no_volume = args.no_cpu_baseline = True
