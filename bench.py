@@ -746,7 +746,7 @@ def serving_bench(seconds=3.0, rate=200.0, seed=4000, workers=8):
     return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s ({n} requests): 85 % 2-D n^2, n FFT-smooth in "
                         "[128, 1024]; 15 % 3-D n^3, n in {64, 80, 96, 128}; ER / HIO 0.9 / RAAR 0.87 uniform; "
                         "100-1000 iterations; static support; random-phase starts",
-            "requests": n, "shapes": len(shapes),
+            "requests": n, "shapes": len({s for _, s, _, _, _ in trace}),
             "served": {"latency": serve.latency_summary(live), "makespan_s": t_live, "recons_per_s": n / t_live,
                        "batches_created": created_live},
             "served_half_rate": {"offered_req_per_s": rate / 2, "requests": int(keep.sum()),
